@@ -1,0 +1,339 @@
+"""fg2_amd -- MI355X (gfx950) implementation of fg2's time-step path.
+
+Python host mirror of the reference's driver API (src/fg2.h:70-84) over the
+C ABI of libfg2_amd.so (include/fg2_amd.h).  The reference keeps one
+simulation in `namespace global`; here a `Run` holds it:
+
+    setup(n1, n2)           -> Run(module, n1, n2)        src/setup.cpp:32
+    config()                -> Run.config()               src/config.cpp:24
+    para("name=val")        -> Run.para("name=val")       src/para.cpp:25
+    getdt()                 -> Run.getdt()                src/getdt.cpp:22
+    step(t, dt)             -> Run.step(t, dt)            src/step.cpp:22
+    solve(t, t_final, ...)  -> Run.solve(t, t_final)      src/solve.cpp:25
+    Euler/bcond/kick/drift  -> Run.euler/bcond/kick/drift src/fg2.h:81-84
+
+All arithmetic runs in the HIP library; this module only moves host arrays
+and calls the C ABI.  Importing it raises if the library is not built:
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfg2_amd.so")
+
+NVAR = {"hydro": 4, "hydro-isot": 3, "disk": 6}
+FIELDS = {
+    "hydro": ("lnd", "u1", "u2", "lne"),
+    "hydro-isot": ("lnd", "u1", "u2"),
+    "disk": ("lnd", "ur", "uz", "Omg", "lne", "lner"),
+}
+
+OK, ERR_NAN, ERR_HIP, ERR_SHAPE, ERR_ARG, ERR_NOMEM = range(6)
+PATH_FUSED, PATH_PLAIN = 0, 1
+DT_CLOSED, DT_CFL = 0, 1
+
+
+class Fg2Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fg2 error {code}: {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m fg2_amd.build` "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, D, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_float
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    DP = ctypes.POINTER(ctypes.c_double)
+    FP = ctypes.POINTER(ctypes.c_float)
+    IP = ctypes.POINTER(ctypes.c_int)
+    LP = ctypes.POINTER(ctypes.c_long)
+    SP = ctypes.POINTER(ctypes.c_size_t)
+    sig = {
+        "fg2_last_error": (ctypes.c_char_p, []),
+        "fg2_setup": (I, [PP, ctypes.c_char_p, I, I, SP]),
+        "fg2_free": (None, [P]),
+        "fg2_set_geometry": (I, [P, D, D, I, I, D]),
+        "fg2_get_geometry": (I, [P, DP]),
+        "fg2_config": (I, [P]),
+        "fg2_para": (ctypes.c_char_p, [P, ctypes.c_char_p]),
+        "fg2_get_param": (I, [P, ctypes.c_char_p, DP]),
+        "fg2_seed": (I, [P, ctypes.c_ulonglong]),
+        "fg2_layout": (I, [P, IP, LP]),
+        "fg2_u": (P, [P]),
+        "fg2_v": (P, [P]),
+        "fg2_stream": (P, [P]),
+        "fg2_upload": (I, [P, FP]),
+        "fg2_download": (I, [P, FP]),
+        "fg2_upload_v": (I, [P, FP]),
+        "fg2_download_v": (I, [P, FP]),
+        "fg2_euler": (I, [P, P, F]),
+        "fg2_bcond": (I, [P, P, I, I]),
+        "fg2_kick": (I, [P, P, P, F, F]),
+        "fg2_drift": (I, [P, P, P, F]),
+        "fg2_getdt": (D, [P]),
+        "fg2_getdt_cfl": (I, [P, DP]),
+        "fg2_set_path": (I, [P, I]),
+        "fg2_set_dt_mode": (I, [P, I]),
+        "fg2_set_tile": (I, [P, I, I]),
+        "fg2_get_tile": (I, [P, IP, IP]),
+        "fg2_step": (I, [P, D, D]),
+        "fg2_step_async": (I, [P, D, D]),
+        "fg2_check": (I, [P]),
+        "fg2_solve": (I, [P, DP, D, IP]),
+        "fg2_sums": (I, [P, DP]),
+        "fg2_perf_model": (I, [P, DP, DP, DP]),
+        "fg2_batch_create": (I, [PP, PP, I, I, I]),
+        "fg2_batch_free": (None, [P]),
+        "fg2_batch_step_async": (I, [P, DP]),
+        "fg2_batch_step": (I, [P, DP]),
+        "fg2_batch_graph_run": (I, [P, DP, I, I]),
+        "fg2_batch_stream": (P, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+# C-ABI symbols declared in include/fg2_amd.h (checked by the CPU tests)
+EXPORTS = [n for n in dir(lib) if n.startswith("fg2_")]
+
+
+def _check(code):
+    if code != OK:
+        raise Fg2Error(code, lib.fg2_last_error().decode())
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Run:
+    """One simulation on the current HIP device (the reference's global state)."""
+
+    def __init__(self, module, n1, n2, params=None, geometry=None, config=True,
+                 seed=0, path=PATH_FUSED, tile=None):
+        if module not in NVAR:
+            raise ValueError(f"unknown module {module!r}")
+        self.module, self.n1, self.n2 = module, int(n1), int(n2)
+        self.nv = NVAR[module]
+        h = ctypes.c_void_p()
+        nbytes = ctypes.c_size_t()
+        _check(lib.fg2_setup(ctypes.byref(h), module.encode(), self.n1, self.n2,
+                             ctypes.byref(nbytes)))
+        self.h = h
+        self.bytes = nbytes.value
+        if geometry is not None:
+            self.set_geometry(**geometry)
+        for k, v in (params or {}).items():
+            self.para(f"{k}={v!r}")
+        if config:
+            self.config()
+        _check(lib.fg2_seed(self.h, seed))
+        self.set_path(path)
+        if tile is not None:
+            self.set_tile(*tile)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.fg2_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.n1, self.n2, self.nv)
+
+    # -- set-up -----------------------------------------------------------
+    def set_geometry(self, l1=1.0, l2=1.0, p1=1, p2=1, c=0.5):
+        _check(lib.fg2_set_geometry(self.h, l1, l2, int(p1), int(p2), c))
+
+    def geometry(self):
+        out = np.zeros(7)
+        _check(lib.fg2_get_geometry(self.h, _dp(out)))
+        return dict(zip(("l1", "l2", "p1", "p2", "c", "Delta1", "Delta2"), out.tolist()))
+
+    def config(self):
+        _check(lib.fg2_config(self.h))
+
+    def para(self, assignment):
+        r = lib.fg2_para(self.h, assignment.encode())
+        if r is None:
+            raise KeyError(lib.fg2_last_error().decode())
+        return r.decode()
+
+    def get_param(self, name):
+        v = ctypes.c_double()
+        _check(lib.fg2_get_param(self.h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def layout(self):
+        nv, s = ctypes.c_int(), ctypes.c_long()
+        _check(lib.fg2_layout(self.h, ctypes.byref(nv), ctypes.byref(s)))
+        return nv.value, s.value
+
+    def u_ptr(self):
+        return lib.fg2_u(self.h)
+
+    def v_ptr(self):
+        return lib.fg2_v(self.h)
+
+    def set_path(self, path):
+        _check(lib.fg2_set_path(self.h, path))
+
+    def set_dt_mode(self, mode):
+        _check(lib.fg2_set_dt_mode(self.h, mode))
+
+    def set_tile(self, tile_h, waves=1):
+        _check(lib.fg2_set_tile(self.h, int(tile_h), int(waves)))
+
+    def tile(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib.fg2_get_tile(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    # -- state ------------------------------------------------------------
+    def upload(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.shape)
+        _check(lib.fg2_upload(self.h, _fp(x)))
+
+    def download(self):
+        x = np.empty(self.shape, np.float32)
+        _check(lib.fg2_download(self.h, _fp(x)))
+        return x
+
+    def upload_v(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.shape)
+        _check(lib.fg2_upload_v(self.h, _fp(x)))
+
+    def download_v(self):
+        x = np.empty(self.shape, np.float32)
+        _check(lib.fg2_download_v(self.h, _fp(x)))
+        return x
+
+    # -- device operations (src/fg2.h:81-84) ------------------------------
+    def euler(self, dt):
+        _check(lib.fg2_euler(self.h, self.u_ptr(), dt))
+
+    def bcond(self, p1=None, p2=None):
+        g = self.geometry()
+        p1 = int(g["p1"]) if p1 is None else p1
+        p2 = int(g["p2"]) if p2 is None else p2
+        _check(lib.fg2_bcond(self.h, self.u_ptr(), p1, p2))
+
+    def kick(self, beta, t=0.0):
+        _check(lib.fg2_kick(self.h, self.v_ptr(), self.u_ptr(), t, beta))
+
+    def drift(self, dt):
+        _check(lib.fg2_drift(self.h, self.u_ptr(), self.v_ptr(), dt))
+
+    # -- integration (src/fg2.h:77-79) ------------------------------------
+    def getdt(self):
+        return lib.fg2_getdt(self.h)
+
+    def getdt_cfl(self):
+        d = ctypes.c_double()
+        _check(lib.fg2_getdt_cfl(self.h, ctypes.byref(d)))
+        return d.value
+
+    def step(self, t, dt):
+        """One RK3 step; returns the status code (0, or ERR_NAN) like step()."""
+        code = lib.fg2_step(self.h, t, dt)
+        if code not in (OK, ERR_NAN):
+            _check(code)
+        return code
+
+    def step_async(self, t, dt):
+        _check(lib.fg2_step_async(self.h, t, dt))
+
+    def check(self):
+        code = lib.fg2_check(self.h)
+        if code not in (OK, ERR_NAN):
+            _check(code)
+        return code
+
+    def run(self, t, dt, n):
+        """n steps of fixed dt, one NaN check at the end."""
+        for k in range(n):
+            self.step_async(t + k * dt, dt)
+        return self.check()
+
+    def solve(self, t, t_final):
+        tt = ctypes.c_double(t)
+        m = ctypes.c_int()
+        code = lib.fg2_solve(self.h, ctypes.byref(tt), t_final, ctypes.byref(m))
+        if code not in (OK, ERR_NAN):
+            _check(code)
+        return code, tt.value, m.value
+
+    def sums(self):
+        out = np.zeros(4)
+        _check(lib.fg2_sums(self.h, _dp(out)))
+        return out
+
+    def perf_model(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib.fg2_perf_model(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"flops": a.value, "bits": b.value, "alg_bytes_per_cell": c.value}
+
+    def stream(self):
+        return lib.fg2_stream(self.h)
+
+
+class Batch:
+    """Runs of one module advanced together by persistent launches."""
+
+    def __init__(self, runs, waves=1, tile_h=0):
+        self.runs = list(runs)
+        arr = (ctypes.c_void_p * len(self.runs))(*[r.h.value for r in self.runs])
+        h = ctypes.c_void_p()
+        _check(lib.fg2_batch_create(ctypes.byref(h), arr, len(self.runs), waves, tile_h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.fg2_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, dts):
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        code = lib.fg2_batch_step(self.h, _dp(d))
+        if code not in (OK, ERR_NAN):
+            _check(code)
+        return code
+
+    def step_async(self, dts):
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        _check(lib.fg2_batch_step_async(self.h, _dp(d)))
+
+    def graph_run(self, dts, steps, replays=1):
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        _check(lib.fg2_batch_graph_run(self.h, _dp(d), steps, replays))
+
+    def stream(self):
+        return lib.fg2_batch_stream(self.h)

@@ -1,0 +1,71 @@
+"""Build the gfx950 shared library libfg2_amd.so in-tree with hipcc.
+
+    python -m fg2_amd.build          # or fg2_amd.build.build()
+
+Flags: -O3 for gfx950 only; packed-FP32 instruction selection is switched
+off (-Xclang -target-feature -Xclang -packed-fp32-ops), so the compiler
+emits scalar v_fma_f32 / v_add_f32 for the FP32 stencil arithmetic.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libfg2_amd.so")
+SOURCES = ["kernels.hip", "capi.hip"]
+HEADERS = ["fg2_layout.h", "physics.h", "launch.h"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+    "-I", CSRC, "-I", os.path.join(ROOT, "include"),
+]
+
+
+def _objects():
+    return [os.path.join(CSRC, s.replace(".hip", ".o")) for s in SOURCES]
+
+
+def build(verbose=False, jobs=2):
+    """Compile every source (always) and link libfg2_amd.so."""
+    procs = []
+    for src, obj in zip(SOURCES, _objects()):
+        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                            stderr=subprocess.STDOUT, text=True)))
+        if len(procs) >= jobs:
+            _wait(procs.pop(0))
+    while procs:
+        _wait(procs.pop(0))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *_objects(), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def _wait(item):
+    cmd, p = item
+    out, _ = p.communicate()
+    if p.returncode != 0:
+        sys.stderr.write(out)
+        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+    if out.strip():
+        # keep warnings visible but short
+        lines = [l for l in out.splitlines() if "packed-fp32-ops" not in l
+                 and "not a recognized feature" not in l]
+        if lines:
+            sys.stderr.write("\n".join(lines[:40]) + "\n")
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

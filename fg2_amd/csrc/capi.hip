@@ -1,0 +1,776 @@
+// capi.hip -- the C-ABI host driver (include/fg2_amd.h).
+//
+// Host counterparts of the reference's setup/config/para/getdt/step/solve
+// (src/setup.cpp, src/config.cpp, src/para.cpp, src/getdt.cpp,
+// src/step.cpp, src/solve.cpp), over the kernels of kernels.hip.  The global
+// state of the reference (namespace global, src/fg2.h:50-58) lives in a run
+// handle, so many runs can coexist (batching, serving).
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "fg2_amd.h"
+#include "launch.h"
+
+using namespace fg2;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if(e_ != hipSuccess)                                                         \
+      return fail(FG2_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));          \
+  } while(0)
+
+// Module parameters and their reference defaults (src/hydro/eqns.cuh:19-24,
+// src/hydro-isot/eqns.cuh:19-25, src/disk/eqns.cuh:19-36), in the order of
+// Params::par used by physics.h.
+struct ParamDef { const char *name; double value; };
+const std::vector<ParamDef> &param_defs(int m)
+{
+  static const std::vector<ParamDef> hydro = {
+    {"gamma", 5.0 / 3.0}, {"nus", 2.0e-4}, {"nub", 0.0}, {"kappa", 5.0e-4}, {"dd", 0.0}};
+  static const std::vector<ParamDef> isot = {
+    {"dd", 0.0}, {"nus", 2.0e-4}, {"nub", 0.0}, {"cs2", 1.0}, {"f", 0.0}, {"k", -8.0}};
+  static const std::vector<ParamDef> disk = {
+    {"M", 1.0}, {"rS", 2.0}, {"rin", 3.0}, {"gamma", 5.0 / 3.0}, {"nus", 0.0}, {"nub", 0.0},
+    {"kappa", 0.0}, {"alpha", 0.01}, {"ar", 1.0e8}, {"es", 1.0}, {"ff", 0.0}, {"dd", 1.0e-4},
+    {"nu", 2.0e-4}, {"ed", 5.0e-4}, {"rd", 5.0e-4}};
+  return m == HYDRO ? hydro : m == HYDRO_ISOT ? isot : disk;
+}
+
+int module_id(const char *name)
+{
+  if(!name) return -1;
+  if(!strcmp(name, "hydro")) return HYDRO;
+  if(!strcmp(name, "hydro-isot")) return HYDRO_ISOT;
+  if(!strcmp(name, "disk")) return DISK;
+  return -1;
+}
+
+// Largest modified wavenumber of D1 and of -D11, and the RK3 stability
+// limits on the imaginary and negative real axes (|1+z+z^2/2+z^3/6| = 1).
+constexpr double KMAX1 = 1.5866979575;
+constexpr double KMAX2 = 1088.0 / 180.0;
+constexpr double RK3_IMAG = 1.7320508075688772;
+constexpr double RK3_REAL = 2.5127453266183286;
+
+// Williamson low-storage RK3 (src/step.cpp:24-26).
+constexpr double RK_ALPHA[3] = {0.0, 1.0 / 3.0, 3.0 / 4.0};
+constexpr double RK_BETA[3]  = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+constexpr double RK_GAMMA[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+
+// SplitMix64 (Steele, Lea, Flood 2014), the seeded stand-in for rand().
+uint64_t splitmix64(uint64_t &state)
+{
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void reset_diag_kernel(DeviceDiag *d, int lam_only)
+{
+  if(!lam_only) d->flags = 0;
+  d->lam_adv_bits = 0;
+  d->lam_diff_bits = 0;
+}
+
+} // namespace
+
+struct fg2_run {
+  int module = 0, nv = 0, n1 = 0, n2 = 0;
+  long s = 0;
+  double l1 = 1.0, l2 = 1.0, c = 0.5;   // src/global.cpp:28-29
+  int p1 = 1, p2 = 1;
+  float Delta1 = 0, Delta2 = 0;
+  bool configured = false;
+  std::vector<double> par;
+  float *buf[3] = {nullptr, nullptr, nullptr};   // allocations
+  float *in[3] = {nullptr, nullptr, nullptr};    // interior pointers
+  int cur = 0;                                   // state lives in in[cur]; v is in[2]
+  DeviceDiag *diag = nullptr;
+  DeviceDiag *host_diag = nullptr;               // pinned readback
+  double *sum_partial = nullptr, *sum_out = nullptr;
+  hipStream_t stream = nullptr;
+  int path = FG2_PATH_FUSED, dt_mode = FG2_DT_CLOSED;
+  FusedShape shape{0, 1};
+  bool tile_auto = true;
+  uint64_t rng = 0;
+  float rand1 = 0, rand2 = 0;
+  bool lam_valid = false;        // diag holds the CFL rates of the current state
+  char para_buf[256];
+  size_t bytes = 0;
+};
+
+namespace {
+
+int auto_tile_h(int n1, int n2)
+{
+  // aim for about 3072 wavefront tiles (12 per CU on 256 CUs)
+  const long nstrips = (n2 + STRIP - 1) / STRIP;
+  long h = ((long)n1 * nstrips + 3071) / 3072;
+  h = std::max(16L, std::min(256L, h));
+  return (int)std::min<long>(h, n1);
+}
+
+Params make_params(const fg2_run *r)
+{
+  Params P{};
+  P.n1 = r->n1; P.n2 = r->n2; P.p1 = r->p1; P.p2 = r->p2;
+  P.Delta1 = r->Delta1; P.Delta2 = r->Delta2;
+  const double D1 = r->Delta1, D2 = r->Delta2;
+  const double c1[3] = {45.0 / 60.0, -9.0 / 60.0, 1.0 / 60.0};
+  const double c2[4] = {-490.0 / 180.0, 270.0 / 180.0, -27.0 / 180.0, 2.0 / 180.0};
+  const double cm[3] = {270.0 / 720.0, -27.0 / 720.0, 2.0 / 720.0};
+  for(int k = 0; k < 3; ++k) {
+    P.a1[k] = (float)(c1[k] / D1);
+    P.a2[k] = (float)(c1[k] / D2);
+    P.m12[k] = (float)(cm[k] / (D1 * D2));
+  }
+  for(int k = 0; k < 4; ++k) {
+    P.b1[k] = (float)(c2[k] / (D1 * D1));
+    P.b2[k] = (float)(c2[k] / (D2 * D2));
+  }
+  for(size_t k = 0; k < r->par.size() && k < 16; ++k) P.par[k] = (float)r->par[k];
+  P.rand1 = r->rand1; P.rand2 = r->rand2;
+  return P;
+}
+
+bool first_order_active(const fg2_run *r)
+{
+  if(r->module == HYDRO_ISOT) return (float)r->par[4] * (float)r->par[5] > 0.0f;  // f k > 0
+  if(r->module == DISK) return (float)r->par[10] != 0.0f;                          // ff != 0
+  return false;
+}
+
+// Draw the two random phases of one Euler() call (src/hydro-isot/Euler.cuh:39-44).
+void randomize(fg2_run *r)
+{
+  if(r->module != HYDRO_ISOT) return;
+  const double u1 = (double)(splitmix64(r->rng) >> 11) * 0x1.0p-53;
+  const double u2 = (double)(splitmix64(r->rng) >> 11) * 0x1.0p-53;
+  r->rand1 = (float)(6.2831853071795865 * u1);
+  r->rand2 = (float)(6.2831853071795865 * u2);
+}
+
+int check_run(const fg2_run *r)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  if(!r->configured) return fail(FG2_ERR_ARG, "run not configured: call fg2_config first");
+  return FG2_OK;
+}
+
+FusedShape shape_of(const fg2_run *r)
+{
+  FusedShape sh = r->shape;
+  if(r->tile_auto || sh.tile_h < 1) sh.tile_h = auto_tile_h(r->n1, r->n2);
+  if(sh.waves != 1 && sh.waves != 2 && sh.waves != 4) sh.waves = 1;
+  return sh;
+}
+
+// Diffusive rate of the Cartesian modules: D_max (k2/D1^2 + k2/D2^2).
+double lam_diff_const(const fg2_run *r)
+{
+  const double k2s = KMAX2 * (1.0 / ((double)r->Delta1 * r->Delta1) +
+                              1.0 / ((double)r->Delta2 * r->Delta2));
+  double dmax = 0.0;
+  if(r->module == HYDRO)
+    dmax = std::max({r->par[1] * 4.0 / 3.0 + r->par[2], (r->par[0] - 1.0) * r->par[3], r->par[4]});
+  else if(r->module == HYDRO_ISOT)
+    dmax = std::max(r->par[1] * 4.0 / 3.0 + r->par[2], r->par[0]);
+  return dmax * k2s;
+}
+
+double dt_from_rates(const fg2_run *r, double lam_adv, double lam_diff_cell)
+{
+  const double k2s = KMAX2 * (1.0 / ((double)r->Delta1 * r->Delta1) +
+                              1.0 / ((double)r->Delta2 * r->Delta2));
+  const double lam_diff = r->module == DISK ? lam_diff_cell * k2s : lam_diff_const(r);
+  double dt = INFINITY;
+  if(lam_adv > 0) dt = std::min(dt, RK3_IMAG / lam_adv);
+  if(lam_diff > 0) dt = std::min(dt, RK3_REAL / lam_diff);
+  return r->c * dt;
+}
+
+double bits_to_double(unsigned long long b)
+{
+  double d;
+  memcpy(&d, &b, sizeof d);
+  return d;
+}
+
+int substages(fg2_run *r, double t, double dt, int cfl)
+{
+  const Params P = make_params(r);
+  (void)t;  // the reference passes t + alpha dt to kick, which ignores it
+  if(first_order_active(r))
+    HIP_TRY(launch_euler(r->module, r->in[r->cur], r->s, P, (float)dt, r->stream));
+  if(r->path == FG2_PATH_PLAIN) {
+    float *u = r->in[r->cur], *v = r->in[2];
+    for(int k = 0; k < 3; ++k) {
+      HIP_TRY(launch_bcond(r->module, u, r->s, P, r->p1, r->p2, r->stream));
+      HIP_TRY(launch_kick(r->module, v, u, r->s, P, (float)RK_BETA[k], r->stream));
+      HIP_TRY(launch_drift(r->module, u, v, r->s, P, (float)(dt * RK_GAMMA[k]), r->diag, r->stream));
+    }
+    r->lam_valid = false;
+    return FG2_OK;
+  }
+  const FusedShape sh = shape_of(r);
+  if(cfl) reset_diag_kernel<<<1, 1, 0, r->stream>>>(r->diag, 1);
+  for(int k = 0; k < 3; ++k) {
+    HIP_TRY(launch_substage(r->module, k, r->in[r->cur], r->in[r->cur ^ 1], r->in[2], r->s, P,
+                            (float)RK_BETA[k], (float)(dt * RK_GAMMA[k]), sh, cfl, r->diag,
+                            r->stream));
+    r->cur ^= 1;
+  }
+  r->lam_valid = cfl != 0;
+  return FG2_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *fg2_last_error(void) { return g_error.c_str(); }
+
+int fg2_setup(fg2_run **out, const char *module, int n1, int n2, size_t *bytes)
+{
+  if(!out) return fail(FG2_ERR_ARG, "NULL output handle");
+  *out = nullptr;
+  const int m = module_id(module);
+  if(m < 0) return fail(FG2_ERR_ARG, "unknown module \"%s\"", module ? module : "(null)");
+  if(n1 < HALF || n2 < HALF) return fail(FG2_ERR_SHAPE, "grid %d x %d: both sides must be >= %d", n1, n2, HALF);
+  fg2_run *r = new(std::nothrow) fg2_run;
+  if(!r) return fail(FG2_ERR_NOMEM, "host allocation failed");
+  r->module = m; r->nv = nvar_of(m); r->n1 = n1; r->n2 = n2;
+  r->s = row_stride(n2, r->nv);
+  for(auto &p : param_defs(m)) r->par.push_back(p.value);
+  const long nf = array_floats(n1, n2, r->nv);
+  const long off = interior_offset(n2, r->nv);
+  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+  for(int k = 0; k < 3 && e == hipSuccess; ++k) {
+    e = hipMalloc((void **)&r->buf[k], nf * sizeof(float));
+    if(e == hipSuccess) {
+      r->in[k] = r->buf[k] + off;
+      // -FLT_MAX everywhere, as src/setup.cpp:80-83
+      const float neg = -FLT_MAX;
+      unsigned int bits;
+      memcpy(&bits, &neg, 4);
+      e = hipMemsetD32Async((hipDeviceptr_t)r->buf[k], (int)bits, nf, r->stream);
+    }
+  }
+  if(e == hipSuccess) e = hipMalloc((void **)&r->diag, sizeof(DeviceDiag));
+  if(e == hipSuccess) e = hipMemsetAsync(r->diag, 0, sizeof(DeviceDiag), r->stream);
+  if(e == hipSuccess) e = hipHostMalloc((void **)&r->host_diag, sizeof(DeviceDiag), hipHostMallocDefault);
+  if(e == hipSuccess) e = hipMalloc((void **)&r->sum_partial, 4 * (size_t)n1 * sizeof(double));
+  if(e == hipSuccess) e = hipMalloc((void **)&r->sum_out, 4 * sizeof(double));
+  if(e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if(e != hipSuccess) {
+    const int code = (e == hipErrorOutOfMemory) ? FG2_ERR_NOMEM : FG2_ERR_HIP;
+    fg2_free(r);
+    return fail(code, "setup %d x %d: %s", n1, n2, hipGetErrorString(e));
+  }
+  r->bytes = 3 * nf * sizeof(float);
+  if(bytes) *bytes = r->bytes;
+  *out = r;
+  return FG2_OK;
+}
+
+void fg2_free(fg2_run *r)
+{
+  if(!r) return;
+  if(r->stream) (void)hipStreamSynchronize(r->stream);
+  for(int k = 0; k < 3; ++k) if(r->buf[k]) (void)hipFree(r->buf[k]);
+  if(r->diag) (void)hipFree(r->diag);
+  if(r->host_diag) (void)hipHostFree(r->host_diag);
+  if(r->sum_partial) (void)hipFree(r->sum_partial);
+  if(r->sum_out) (void)hipFree(r->sum_out);
+  if(r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+}
+
+int fg2_set_geometry(fg2_run *r, double l1, double l2, int p1, int p2, double c)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  if(!(l1 > 0) || !(l2 > 0) || !(c > 0)) return fail(FG2_ERR_ARG, "l1, l2, c must be > 0");
+  // global::l1, l2, c are R (float) in the reference
+  r->l1 = (float)l1; r->l2 = (float)l2; r->c = (float)c;
+  r->p1 = p1 ? 1 : 0; r->p2 = p2 ? 1 : 0;
+  if(r->configured) {
+    r->Delta1 = (float)r->l1 / (float)r->n1;
+    r->Delta2 = (float)r->l2 / (float)r->n2;
+  }
+  return FG2_OK;
+}
+
+int fg2_get_geometry(const fg2_run *r, double *out)
+{
+  if(!r || !out) return fail(FG2_ERR_ARG, "NULL argument");
+  out[0] = r->l1; out[1] = r->l2; out[2] = r->p1; out[3] = r->p2; out[4] = r->c;
+  out[5] = r->Delta1; out[6] = r->Delta2;
+  return FG2_OK;
+}
+
+int fg2_config(fg2_run *r)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  // module settings(): src/hydro-isot/config.h:21-26, src/disk/config.h:21-34
+  if(r->module == HYDRO_ISOT) r->l1 = r->l2 = (float)(2.0 * M_PI);
+  if(r->module == DISK) {
+    r->l2 = (float)M_PI;
+    const float hd2 = (float)(0.5 * r->l2 / r->n2);
+    const float hd1 = (float)std::log(hd2 + std::sqrt(hd2 * (double)hd2 + 1.0));
+    r->l1 = (float)(2.0 * hd1 * r->n1);
+    r->p1 = 0; r->p2 = 0;
+  }
+  // Delta = l / n in R (src/config.cpp:39)
+  r->Delta1 = (float)r->l1 / (float)r->n1;
+  r->Delta2 = (float)r->l2 / (float)r->n2;
+  r->configured = true;
+  return FG2_OK;
+}
+
+const char *fg2_para(fg2_run *r, const char *in)
+{
+  if(!r || !in) { fail(FG2_ERR_ARG, "NULL argument"); return nullptr; }
+  while(*in == '-') ++in;                       // src/para.cpp:31
+  const char *eq = strchr(in, '=');
+  if(!eq) { fail(FG2_ERR_ARG, "\"%s\" is not name=val", in); return nullptr; }
+  const std::string name(in, eq - in);
+  const auto &defs = param_defs(r->module);
+  for(size_t k = 0; k < defs.size(); ++k)
+    if(name == defs[k].name) {
+      const float val = (float)atof(eq + 1);     // stored as R (src/para.cpp:33)
+      r->par[k] = val;
+      snprintf(r->para_buf, sizeof r->para_buf, "%s = %g", name.c_str(), (double)val);
+      return r->para_buf;
+    }
+  fail(FG2_ERR_ARG, "module has no parameter \"%s\"", name.c_str());
+  return nullptr;
+}
+
+int fg2_get_param(const fg2_run *r, const char *name, double *value)
+{
+  if(!r || !name || !value) return fail(FG2_ERR_ARG, "NULL argument");
+  const auto &defs = param_defs(r->module);
+  for(size_t k = 0; k < defs.size(); ++k)
+    if(!strcmp(name, defs[k].name)) { *value = r->par[k]; return FG2_OK; }
+  return fail(FG2_ERR_ARG, "module has no parameter \"%s\"", name);
+}
+
+int fg2_seed(fg2_run *r, unsigned long long seed)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  r->rng = seed;
+  return FG2_OK;
+}
+
+int fg2_layout(const fg2_run *r, int *nvar, long *stride)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  if(nvar) *nvar = r->nv;
+  if(stride) *stride = r->s;
+  return FG2_OK;
+}
+
+float *fg2_u(fg2_run *r) { return r ? r->in[r->cur] : nullptr; }
+float *fg2_v(fg2_run *r) { return r ? r->in[2] : nullptr; }
+void *fg2_stream(fg2_run *r) { return r ? (void *)r->stream : nullptr; }
+
+static int copy_interior(fg2_run *r, float *dev, float *host, bool to_dev)
+{
+  if(!r || !host) return fail(FG2_ERR_ARG, "NULL argument");
+  const size_t hp = (size_t)r->n2 * r->nv * sizeof(float);
+  const size_t dp = (size_t)r->s * sizeof(float);
+  if(to_dev) HIP_TRY(hipMemcpy2DAsync(dev, dp, host, hp, hp, r->n1, hipMemcpyHostToDevice, r->stream));
+  else       HIP_TRY(hipMemcpy2DAsync(host, hp, dev, dp, hp, r->n1, hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return FG2_OK;
+}
+
+int fg2_upload(fg2_run *r, const float *h)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  r->lam_valid = false;
+  return copy_interior(r, r->in[r->cur], (float *)h, true);
+}
+int fg2_download(fg2_run *r, float *h) { return r ? copy_interior(r, r->in[r->cur], h, false) : fail(FG2_ERR_ARG, "NULL run"); }
+int fg2_upload_v(fg2_run *r, const float *h) { return r ? copy_interior(r, r->in[2], (float *)h, true) : fail(FG2_ERR_ARG, "NULL run"); }
+int fg2_download_v(fg2_run *r, float *h) { return r ? copy_interior(r, r->in[2], h, false) : fail(FG2_ERR_ARG, "NULL run"); }
+
+int fg2_euler(fg2_run *r, float *u, float dt)
+{
+  if(int e = check_run(r)) return e;
+  if(!u) return fail(FG2_ERR_ARG, "NULL u");
+  randomize(r);
+  if(!first_order_active(r)) return FG2_OK;
+  HIP_TRY(launch_euler(r->module, u, r->s, make_params(r), dt, r->stream));
+  return FG2_OK;
+}
+
+int fg2_bcond(fg2_run *r, float *u, int p1, int p2)
+{
+  if(int e = check_run(r)) return e;
+  if(!u) return fail(FG2_ERR_ARG, "NULL u");
+  HIP_TRY(launch_bcond(r->module, u, r->s, make_params(r), p1 ? 1 : 0, p2 ? 1 : 0, r->stream));
+  return FG2_OK;
+}
+
+int fg2_kick(fg2_run *r, float *v, const float *u, float t, float beta)
+{
+  (void)t;
+  if(int e = check_run(r)) return e;
+  if(!u || !v) return fail(FG2_ERR_ARG, "NULL array");
+  HIP_TRY(launch_kick(r->module, v, u, r->s, make_params(r), beta, r->stream));
+  return FG2_OK;
+}
+
+int fg2_drift(fg2_run *r, float *u, const float *v, float dt)
+{
+  if(int e = check_run(r)) return e;
+  if(!u || !v) return fail(FG2_ERR_ARG, "NULL array");
+  HIP_TRY(launch_drift(r->module, u, v, r->s, make_params(r), dt, r->diag, r->stream));
+  return FG2_OK;
+}
+
+double fg2_getdt(const fg2_run *r)
+{
+  if(!r) return 0.0;
+  // (E)c / (n1 / l1 + n2 / l2) with l, c in R (src/getdt.cpp:26)
+  const float a = (float)r->n1 / (float)r->l1, b = (float)r->n2 / (float)r->l2;
+  return (double)(float)r->c / (double)(a + b);
+}
+
+int fg2_getdt_cfl(fg2_run *r, double *dt)
+{
+  if(int e = check_run(r)) return e;
+  if(!dt) return fail(FG2_ERR_ARG, "NULL dt");
+  if(!r->lam_valid) {
+    reset_diag_kernel<<<1, 1, 0, r->stream>>>(r->diag, 1);
+    HIP_TRY(launch_cfl(r->module, r->in[r->cur], r->s, make_params(r), r->diag, r->stream));
+    r->lam_valid = true;
+  }
+  HIP_TRY(hipMemcpyAsync(r->host_diag, r->diag, sizeof(DeviceDiag), hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  *dt = dt_from_rates(r, bits_to_double(r->host_diag->lam_adv_bits),
+                      bits_to_double(r->host_diag->lam_diff_bits));
+  return FG2_OK;
+}
+
+int fg2_set_path(fg2_run *r, int path)
+{
+  if(!r || (path != FG2_PATH_FUSED && path != FG2_PATH_PLAIN)) return fail(FG2_ERR_ARG, "bad path");
+  r->path = path;
+  return FG2_OK;
+}
+
+int fg2_set_dt_mode(fg2_run *r, int mode)
+{
+  if(!r || (mode != FG2_DT_CLOSED && mode != FG2_DT_CFL)) return fail(FG2_ERR_ARG, "bad dt mode");
+  r->dt_mode = mode;
+  return FG2_OK;
+}
+
+int fg2_set_tile(fg2_run *r, int tile_h, int waves)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  if(waves != 1 && waves != 2 && waves != 4) return fail(FG2_ERR_SHAPE, "waves must be 1, 2 or 4");
+  if(tile_h < 0) return fail(FG2_ERR_SHAPE, "tile_h must be >= 0");
+  r->tile_auto = tile_h == 0;
+  r->shape.tile_h = tile_h;
+  r->shape.waves = waves;
+  return FG2_OK;
+}
+
+int fg2_get_tile(const fg2_run *r, int *tile_h, int *waves)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  const FusedShape sh = shape_of(r);
+  if(tile_h) *tile_h = sh.tile_h;
+  if(waves) *waves = sh.waves;
+  return FG2_OK;
+}
+
+int fg2_step_async(fg2_run *r, double t, double dt)
+{
+  if(int e = check_run(r)) return e;
+  randomize(r);
+  return substages(r, t, dt, r->dt_mode == FG2_DT_CFL);
+}
+
+int fg2_check(fg2_run *r)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  HIP_TRY(hipMemcpyAsync(r->host_diag, r->diag, sizeof(DeviceDiag), hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  if(r->host_diag->flags & 1u) {
+    reset_diag_kernel<<<1, 1, 0, r->stream>>>(r->diag, 0);
+    (void)hipStreamSynchronize(r->stream);
+    r->lam_valid = false;
+    return fail(FG2_ERR_NAN, "the state contains NaN");
+  }
+  return FG2_OK;
+}
+
+int fg2_step(fg2_run *r, double t, double dt)
+{
+  if(int e = fg2_step_async(r, t, dt)) return e;
+  return fg2_check(r);
+}
+
+int fg2_solve(fg2_run *r, double *t, double t_final, int *steps)
+{
+  if(int e = check_run(r)) return e;
+  if(!t) return fail(FG2_ERR_ARG, "NULL t");
+  int m = 0;
+  double tt = *t;
+  int status = FG2_OK;
+  while(tt < t_final) {                                       // src/solve.cpp:37-49
+    double dt0;
+    if(r->dt_mode == FG2_DT_CFL) {
+      if(int e = fg2_getdt_cfl(r, &dt0)) { status = e; break; }
+    } else dt0 = fg2_getdt(r);
+    const double dt = std::min(t_final - tt, dt0);
+    if(!(dt > 0)) { status = fail(FG2_ERR_ARG, "non-positive time step %g", dt); break; }
+    ++m;
+    if(int e = fg2_step(r, tt, dt)) { status = e; break; }
+    tt = std::min(tt + dt, t_final);
+  }
+  *t = tt;
+  if(steps) *steps = m;
+  return status;
+}
+
+int fg2_sums(fg2_run *r, double *out)
+{
+  if(int e = check_run(r)) return e;
+  if(!out) return fail(FG2_ERR_ARG, "NULL out");
+  HIP_TRY(launch_sums(r->module, r->in[r->cur], r->s, make_params(r), r->sum_partial, r->sum_out, r->stream));
+  HIP_TRY(hipMemcpyAsync(out, r->sum_out, 4 * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return FG2_OK;
+}
+
+int fg2_perf_model(const fg2_run *r, double *flops, double *bits, double *alg)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  static const int eqns_flop[3] = {289, 214, 733};   // src/*/config.h
+  const double n = (double)r->n1 * r->n2;
+  const double m1 = r->n1 + ORDER, m2 = r->n2 + ORDER;
+  if(flops) *flops = 3.0 * n * (eqns_flop[r->module] + r->nv * 2.0);           // src/config.cpp:33
+  if(bits) *bits = 3.0 * (m1 * m2 + n * 5.0 + (m1 + m2) * 2.0 * ORDER) * r->nv * sizeof(float) * 8;
+  if(alg) *alg = 10.0 * r->nv * sizeof(float);
+  return FG2_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+struct fg2_batch {
+  int module = 0, nruns = 0, waves = 1, grid = 0, total = 0;
+  std::vector<fg2_run *> runs;
+  std::vector<int> tile_h;
+  BatchRun *dev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};  // [stage][parity]
+  std::vector<BatchRun> host[3][2];
+  int *counters = nullptr;   // one per stage
+  hipStream_t stream = nullptr;
+  int parity = 0;
+};
+
+namespace {
+
+__global__ void zero_ints_kernel(int *p, int n)
+{
+  if((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+int batch_fill(fg2_batch *b, const double *dts)
+{
+  // stage k reads u[cur ^ (k & 1)] and writes u[cur ^ 1 ^ (k & 1)] of each
+  // run, where cur is the run's state buffer at the start of the step
+  for(int par = 0; par < 2; ++par)
+    for(int k = 0; k < 3; ++k) {
+      auto &H = b->host[k][par];
+      H.resize(b->nruns);
+      int begin = 0;
+      for(int i = 0; i < b->nruns; ++i) {
+        fg2_run *r = b->runs[i];
+        const int cur = r->cur ^ par;
+        BatchRun &R = H[i];
+        R.uin = r->in[cur ^ (k & 1)];
+        R.uout = r->in[cur ^ 1 ^ (k & 1)];
+        R.v = r->in[2];
+        R.s = r->s;
+        R.P = make_params(r);
+        R.tile_h = b->tile_h[i];
+        R.nstrips = (r->n2 + STRIP - 1) / STRIP;
+        R.ntiles = R.nstrips * ((r->n1 + R.tile_h - 1) / R.tile_h);
+        R.tile_begin = begin;
+        begin += R.ntiles;
+        R.beta = (float)RK_BETA[k];
+        R.gdt = (float)(dts[i] * RK_GAMMA[k]);
+        R.diag = r->diag;
+      }
+      b->total = begin;
+      HIP_TRY(hipMemcpyAsync(b->dev[k][par], H.data(), b->nruns * sizeof(BatchRun),
+                             hipMemcpyHostToDevice, b->stream));
+    }
+  return FG2_OK;
+}
+
+int batch_launch(fg2_batch *b, int par)
+{
+  zero_ints_kernel<<<1, 64, 0, b->stream>>>(b->counters, 3);
+  for(int k = 0; k < 3; ++k)
+    HIP_TRY(launch_batch_substage(b->module, k, b->dev[k][par], b->nruns, b->total, b->waves,
+                                  b->grid, b->counters + k, 0, b->stream));
+  return FG2_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int fg2_batch_create(fg2_batch **out, fg2_run *const *runs, int nruns, int waves, int tile_h)
+{
+  if(!out || !runs || nruns < 1) return fail(FG2_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  if(waves != 1 && waves != 2 && waves != 4) return fail(FG2_ERR_SHAPE, "waves must be 1, 2 or 4");
+  fg2_batch *b = new(std::nothrow) fg2_batch;
+  if(!b) return fail(FG2_ERR_NOMEM, "host allocation failed");
+  b->module = runs[0]->module;
+  b->nruns = nruns;
+  b->waves = waves;
+  for(int i = 0; i < nruns; ++i) {
+    if(int e = check_run(runs[i])) { delete b; return e; }
+    if(runs[i]->module != b->module) { delete b; return fail(FG2_ERR_ARG, "a batch holds runs of one module"); }
+    b->runs.push_back(runs[i]);
+    b->tile_h.push_back(tile_h > 0 ? std::min(tile_h, runs[i]->n1) : shape_of(runs[i]).tile_h);
+  }
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  for(int k = 0; k < 3 && e == hipSuccess; ++k)
+    for(int p = 0; p < 2 && e == hipSuccess; ++p)
+      e = hipMalloc((void **)&b->dev[k][p], nruns * sizeof(BatchRun));
+  if(e == hipSuccess) e = hipMalloc((void **)&b->counters, 4 * sizeof(int));
+  int dev = 0, cus = 256;
+  if(e == hipSuccess) e = hipGetDevice(&dev);
+  if(e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if(e != hipSuccess) { fg2_batch_free(b); return fail(FG2_ERR_HIP, "batch setup: %s", hipGetErrorString(e)); }
+  // persistent grid: 16 wavefronts per CU
+  b->grid = cus * 16 / waves;
+  *out = b;
+  return FG2_OK;
+}
+
+void fg2_batch_free(fg2_batch *b)
+{
+  if(!b) return;
+  if(b->stream) (void)hipStreamSynchronize(b->stream);
+  for(int k = 0; k < 3; ++k)
+    for(int p = 0; p < 2; ++p) if(b->dev[k][p]) (void)hipFree(b->dev[k][p]);
+  if(b->counters) (void)hipFree(b->counters);
+  if(b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+void *fg2_batch_stream(fg2_batch *b) { return b ? (void *)b->stream : nullptr; }
+
+int fg2_batch_step_async(fg2_batch *b, const double *dts)
+{
+  if(!b || !dts) return fail(FG2_ERR_ARG, "NULL argument");
+  // the host copies of the launch records are rewritten below
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  // the runs' own streams may still hold work (uploads): order after them
+  for(fg2_run *r : b->runs) {
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    randomize(r);
+    if(first_order_active(r))
+      return fail(FG2_ERR_ARG, "batched runs with an active first_order are not supported");
+  }
+  if(int e = batch_fill(b, dts)) return e;
+  if(int e = batch_launch(b, 0)) return e;
+  for(fg2_run *r : b->runs) { r->cur ^= 1; r->lam_valid = false; }
+  return FG2_OK;
+}
+
+int fg2_batch_step(fg2_batch *b, const double *dts)
+{
+  if(int e = fg2_batch_step_async(b, dts)) return e;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  int status = FG2_OK;
+  for(fg2_run *r : b->runs) {
+    const int e = fg2_check(r);
+    if(e && !status) status = e;
+  }
+  return status;
+}
+
+int fg2_batch_graph_run(fg2_batch *b, const double *dts, int steps, int replays)
+{
+  if(!b || !dts || steps < 1 || replays < 0) return fail(FG2_ERR_ARG, "bad graph arguments");
+  for(fg2_run *r : b->runs) {
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    if(first_order_active(r))
+      return fail(FG2_ERR_ARG, "batched runs with an active first_order are not supported");
+  }
+  if(int e = batch_fill(b, dts)) return e;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  hipGraph_t g;
+  hipGraphExec_t ge;
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int err = FG2_OK;
+  for(int k = 0; k < steps && !err; ++k) err = batch_launch(b, k & 1);
+  hipError_t ce = hipStreamEndCapture(b->stream, &g);
+  if(err) return err;
+  if(ce != hipSuccess) return fail(FG2_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+  HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  for(int k = 0; k < replays; ++k) {
+    HIP_TRY(hipGraphLaunch(ge, b->stream));
+    // a replay of an odd number of steps flips every run's state buffer
+    if(steps & 1) {
+      for(fg2_run *r : b->runs) r->cur ^= 1;
+      if(k + 1 < replays) {
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        HIP_TRY(hipGraphExecDestroy(ge));
+        HIP_TRY(hipGraphDestroy(g));
+        return fail(FG2_ERR_ARG, "replaying an odd step count more than once is not supported");
+      }
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for(fg2_run *r : b->runs) {
+    for(long k = 0; k < (long)steps * replays; ++k) randomize(r);
+    r->lam_valid = false;
+  }
+  HIP_TRY(hipGraphExecDestroy(ge));
+  HIP_TRY(hipGraphDestroy(g));
+  return FG2_OK;
+}
+
+} // extern "C"

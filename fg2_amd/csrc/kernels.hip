@@ -1,0 +1,620 @@
+// kernels.hip -- HIP kernels for gfx950: the plain per-operation path
+// (kick, drift, boundary fill, first_order) and the fused RK3 substage.
+//
+// Fused substage, one wavefront per tile of 64 columns x tile_h rows:
+//   - lane j owns column j0 + j; the wave marches down the slow axis (n1)
+//     keeping rows i-3 .. i+4 of its column in VGPRs (an 8-slot ring whose
+//     slot index is a compile-time constant in the 8x unrolled loop, so the
+//     ring never moves data);
+//   - row i+4 is loaded while row i is computed (one row of prefetch);
+//   - fast-axis neighbours j+-1..3 come from a wave-private LDS copy of row
+//     i widened by 3 halo cells per side; the halo cells of every loaded row
+//     are fetched by lanes 0..5 and kept in an 8-row LDS ring;
+//   - the mixed derivative uses g_k(x) = f(i+k,x) - f(i-k,x), staged in LDS,
+//     so D12 = sum_k m_k (g_k(j+k) - g_k(j-k));
+//   - boundaries are index maps (periodic wrap, even mirror on dim 1,
+//     mirror with the module's odd-variable sign on dim 2), which give the
+//     values the reference's ghost fill would (src/bcond.cu:22-86);
+//   - the epilogue does v <- F + beta v and u_out <- u_in + gamma dt v,
+//     skipping the v read in stage 0 (beta = 0) and the v write in stage 2
+//     (the next step's stage 0 ignores v).
+// Wavefronts of one workgroup never share LDS, so they synchronise only at
+// wavefront scope: no s_barrier and no vmcnt drain inside the march.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "launch.h"
+#include "physics.h"
+
+namespace fg2 {
+
+// Compiler barrier for LDS traffic between lanes of one wavefront.  A
+// wavefront's DS instructions execute in order, so no hardware wait is needed.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template<int NV> __device__ __forceinline__ void load_cell(const float *p, float (&x)[NV])
+{
+  if constexpr(NV == 4) {
+    const float4 a = *(const float4 *)p;
+    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+  } else if constexpr(NV == 6) {
+    const float2 a = *(const float2 *)p, b = *(const float2 *)(p + 2), c = *(const float2 *)(p + 4);
+    x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y; x[4] = c.x; x[5] = c.y;
+  } else {
+#pragma unroll
+    for(int q = 0; q < NV; ++q) x[q] = p[q];
+  }
+}
+
+template<int NV> __device__ __forceinline__ void store_cell(float *p, const float (&x)[NV])
+{
+  if constexpr(NV == 4) {
+    *(float4 *)p = make_float4(x[0], x[1], x[2], x[3]);
+  } else if constexpr(NV == 6) {
+    *(float2 *)p = make_float2(x[0], x[1]);
+    *(float2 *)(p + 2) = make_float2(x[2], x[3]);
+    *(float2 *)(p + 4) = make_float2(x[4], x[5]);
+  } else {
+#pragma unroll
+    for(int q = 0; q < NV; ++q) p[q] = x[q];
+  }
+}
+
+// Row index of a possibly-ghost row r (|overflow| <= HALF, n >= HALF):
+// periodic wrap, or the even mirror of Neumann1 (src/bcond.cu:45-57).
+__device__ __forceinline__ int map_row(int r, int n, int periodic)
+{
+  if(r < 0) return periodic ? r + n : -1 - r;
+  if(r >= n) return periodic ? r - n : 2 * n - 1 - r;
+  return r;
+}
+// Column index of a possibly-ghost column c, and the sign the reflective
+// boundary applies to the module's odd variable (src/bcond.cu:59-75).
+__device__ __forceinline__ int map_col(int c, int n, int periodic, float &sign)
+{
+  sign = 1.0f;
+  if(c < 0) { if(periodic) return c + n; sign = -1.0f; return -1 - c; }
+  if(c >= n) { if(periodic) return c - n; sign = -1.0f; return 2 * n - 1 - c; }
+  return c;
+}
+
+// Map a linear workgroup id to a tile id so that the workgroups one XCD
+// receives (ids x, x+8, x+16, ... under round-robin dispatch) cover a
+// contiguous range of tiles, whose neighbouring strips share halo columns
+// in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb)
+{
+  constexpr int NX = 8;
+  const int per = nb / NX, rem = nb % NX;
+  const int x = bid % NX, k = bid / NX;
+  return (x < rem) ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
+// ---------------------------------------------------------------------------
+// Fused substage, one tile.  Shared by the single-run and the batched kernels.
+// ---------------------------------------------------------------------------
+struct TileOut { int bad; float lam_adv, lam_diff; };
+
+// Value of x at lane (lane + k) of the wavefront (ds_bpermute; addr holds
+// ((lane + k) & 63) * 4).  Lanes 3..60 never wrap for |k| <= 3.
+__device__ __forceinline__ float lane_shift(float x, int addr)
+{
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
+}
+
+template<class M, int STAGE>
+__device__ __forceinline__ void process_tile(const float *__restrict__ uin, float *__restrict__ uout,
+                                             float *__restrict__ v, const long s, const Params &P,
+                                             const float beta, const float gdt,
+                                             const int j0, const int i0, const int i1,
+                                             const int cfl, TileOut &res)
+{
+  constexpr int NV = M::NV, ND = M::ND, RING = 9;
+  const int lane = threadIdx.x & 63;
+  const int n1 = P.n1, n2 = P.n2;
+  // lane holds column c; lanes 3..60 with c < n2 produce output
+  const int c = j0 - HALF + lane;
+  const bool out = lane >= HALF && lane < 64 - HALF && c < n2;
+  float sgn;
+  const int cc = (c < n2 + HALF) ? map_col(c, n2, P.p2, sgn) : (sgn = 1.0f, n2 - 1);
+  const float *col = uin + (long)cc * NV;
+  int addr[2][3];                      // bpermute addresses for lane +k / -k
+#pragma unroll
+  for(int k = 1; k <= 3; ++k) {
+    addr[0][k - 1] = ((lane + k) & 63) << 2;
+    addr[1][k - 1] = ((lane - k) & 63) << 2;
+  }
+  const float k1a = 1.5866979575f / P.Delta1, k1b = 1.5866979575f / P.Delta2;
+
+  // register ring: row r lives in slot (r - (i0 - 3)) % 9; rows i-3..i+3
+  // are in use, row i+4 is in flight, row i+5 is being requested
+  float w[RING][NV];
+  // rows past i1 + 2 are never used; clamping keeps every load in bounds
+  auto load_row = [&](int r, float (&dst)[NV]) {
+    load_cell<NV>(col + (long)map_row(min(r, i1 + 2), n1, P.p1) * s, dst);
+  };
+#pragma unroll
+  for(int k = 0; k < RING - 1; ++k) load_row(i0 - HALF + k, w[k]);
+#pragma unroll
+  for(int k = 0; k < RING - 2; ++k) w[k][M::ODD] *= sgn;
+
+  const long o0 = (long)(j0 - HALF + lane) * NV;   // output offset in a row
+  // v ring: row i lives in slot (i - i0) % 3; row i+1 is in flight.  Lanes
+  // without output read a valid column so that every load is unconditional
+  // (a predicated load would end in a register join that waits for it).
+  const float *vcol = v + (long)min(max(c, 0), n2 - 1) * NV;
+  float vr[3][NV];
+  if constexpr(STAGE > 0) load_cell<NV>(vcol + (long)i0 * s, vr[0]);
+
+  for(int base = i0; base < i1; base += RING) {
+#pragma unroll
+    for(int p = 0; p < RING; ++p) {
+      const int i = base + p;
+      if(i >= i1) break;
+      // row i+3 enters use: apply the reflective sign of this lane's column
+      if(i > i0) w[(p + 6) % RING][M::ODD] *= sgn;
+      load_row(i + 5, w[(p + 8) % RING]);
+      if constexpr(STAGE > 0) load_cell<NV>(vcol + (long)min(i + 1, i1 - 1) * s, vr[(p + 1) % 3]);
+      const float (&vo)[NV] = vr[p % 3];
+
+      const float (&cen)[NV] = w[(p + 3) % RING];
+      typename M::P_t x;
+#pragma unroll
+      for(int q = 0; q < NV; ++q) x.c[q] = cen[q];
+#pragma unroll
+      for(int k = 1; k <= 3; ++k) {
+#pragma unroll
+        for(int q = 0; q < NV; ++q) {
+          const float a = w[(p + 3 + k) % RING][q], b = w[(p + 3 - k) % RING][q];
+          x.dv[k - 1][q] = a - b;
+          x.sv[k - 1][q] = a + b;
+          const float e = lane_shift(cen[q], addr[0][k - 1]);
+          const float f = lane_shift(cen[q], addr[1][k - 1]);
+          x.dh[k - 1][q] = e - f;
+          x.sh[k - 1][q] = e + f;
+        }
+#pragma unroll
+        for(int d = 0; d < ND; ++d) {
+          const float g = x.dv[k - 1][M::D12V[d]];    // f(i+k, .) - f(i-k, .)
+          x.mx[k - 1][d] = lane_shift(g, addr[0][k - 1]) - lane_shift(g, addr[1][k - 1]);
+        }
+      }
+      if(out) {
+        float f[NV], vn[NV], uo[NV];
+        M::rhs(P, x, i, c, f);
+#pragma unroll
+        for(int q = 0; q < NV; ++q) {
+          vn[q] = (STAGE == 0) ? f[q] : __builtin_fmaf(beta, vo[q], f[q]);
+          uo[q] = __builtin_fmaf(gdt, vn[q], cen[q]);
+          res.bad |= (uo[q] != uo[q]);
+        }
+        const long o = (long)i * s + o0;
+        if constexpr(STAGE < 2) store_cell<NV>(v + o, vn);
+        store_cell<NV>(uout + o, uo);
+        if constexpr(STAGE == 2) {
+          if(cfl) {
+            res.lam_adv = fmaxf(res.lam_adv, M::lam_adv(P, uo, i, c, k1a, k1b));
+            if constexpr(M::CELL_DIFF) res.lam_diff = fmaxf(res.lam_diff, M::lam_diff(P, uo, i, c));
+          }
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ float wave_max(float x)
+{
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// Fold one tile's NaN flag and CFL rates into the run's diagnostics.
+__device__ __forceinline__ void commit_tile(const TileOut &r, const int cfl, DeviceDiag *diag)
+{
+  const unsigned long long badmask = __ballot(r.bad);
+  const int lane = threadIdx.x & 63;
+  if(badmask && lane == 0) atomicOr(&diag->flags, 1u);
+  if(cfl) {
+    const float la = wave_max(r.lam_adv), ld = wave_max(r.lam_diff);
+    if(lane == 0) {
+      // non-negative doubles order like their bit patterns
+      atomicMax(&diag->lam_adv_bits, (unsigned long long)__double_as_longlong((double)la));
+      atomicMax(&diag->lam_diff_bits, (unsigned long long)__double_as_longlong((double)ld));
+    }
+  }
+}
+
+template<class M, int STAGE, int WPB>
+__global__ __launch_bounds__(64 * WPB)
+void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, float *__restrict__ v,
+                     const long s, const Params P, const float beta, const float gdt,
+                     const int tile_h, const int nstrips, const int ntiles, const int cfl,
+                     DeviceDiag *diag)
+{
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * WPB + wv;
+  if(tile >= ntiles) return;
+  const int strip = tile % nstrips, chunk = tile / nstrips;
+  const int i0 = chunk * tile_h, i1 = min(P.n1, i0 + tile_h);
+  TileOut r{0, 0.0f, 0.0f};
+  process_tile<M, STAGE>(uin, uout, v, s, P, beta, gdt, strip * STRIP, i0, i1, cfl, r);
+  commit_tile(r, cfl, diag);
+}
+
+// Batched substage: a persistent grid of wavefronts pulls tiles of many runs
+// from one atomic counter.  Every tile runs the same process_tile as the
+// single-run kernel, so a run gives the same bits alone and in a batch.
+template<class M, int STAGE, int WPB>
+__global__ __launch_bounds__(64 * WPB)
+void batch_kernel(const BatchRun *__restrict__ runs, const int nruns, const int total_tiles,
+                  int *counter, const int cfl)
+{
+  const int lane = threadIdx.x & 63;
+  for(;;) {
+    int t = 0;
+    if(lane == 0) t = atomicAdd(counter, 1);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+    if(t >= total_tiles) break;
+    // binary search for the run owning tile t (tiles are ordered by run)
+    int lo = 0, hi = nruns - 1;
+    while(lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if(runs[mid].tile_begin <= t) lo = mid; else hi = mid - 1;
+    }
+    const int r = lo;
+    const BatchRun &R = runs[r];
+    const int lt = t - R.tile_begin;
+    const int strip = lt % R.nstrips, chunk = lt / R.nstrips;
+    const int i0 = chunk * R.tile_h, i1 = min(R.P.n1, i0 + R.tile_h);
+    TileOut res{0, 0.0f, 0.0f};
+    process_tile<M, STAGE>(R.uin, R.uout, R.v, R.s, R.P, R.beta, R.gdt, strip * STRIP, i0, i1,
+                           cfl, res);
+    commit_tile(res, cfl, R.diag);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plain path, ghost zones in memory (the reference's operation split).
+// ---------------------------------------------------------------------------
+
+// v <- F(u) + beta v, one thread per cell, stencil read from the ghosted
+// array (src/kick.cu:59-106 computes the same from a shared-memory tile).
+template<class M>
+__global__ __launch_bounds__(256)
+void kick_kernel(float *__restrict__ v, const float *__restrict__ u, const long s,
+                 const Params P, const float beta)
+{
+  constexpr int NV = M::NV, ND = M::ND;
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if(j >= P.n2) return;
+  const float *c = u + (long)i * s + (long)j * NV;
+  typename M::P_t x;
+#pragma unroll
+  for(int q = 0; q < NV; ++q) x.c[q] = c[q];
+#pragma unroll
+  for(int k = 1; k <= 3; ++k) {
+#pragma unroll
+    for(int q = 0; q < NV; ++q) {
+      const float a = c[k * s + q], b = c[-k * s + q];
+      x.dv[k - 1][q] = a - b;
+      x.sv[k - 1][q] = a + b;
+      const float e = c[k * NV + q], f = c[-k * NV + q];
+      x.dh[k - 1][q] = e - f;
+      x.sh[k - 1][q] = e + f;
+    }
+#pragma unroll
+    for(int d = 0; d < ND; ++d) {
+      const int q = M::D12V[d];
+      x.mx[k - 1][d] = (c[k * s + k * NV + q] - c[-k * s + k * NV + q])
+                     - (c[k * s - k * NV + q] - c[-k * s - k * NV + q]);
+    }
+  }
+  float f[NV];
+  M::rhs(P, x, i, j, f);
+  float *o = v + (long)i * s + (long)j * NV;
+#pragma unroll
+  for(int q = 0; q < NV; ++q) o[q] = __builtin_fmaf(beta, o[q], f[q]);
+}
+
+// u <- u + dt v with the sticky NaN flag (src/drift.cu:23-32).
+template<int NV>
+__global__ __launch_bounds__(256)
+void drift_kernel(float *__restrict__ u, const float *__restrict__ v, const long s,
+                  const int n2, const float dt, DeviceDiag *diag)
+{
+  const int i = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if(k >= n2 * NV) return;
+  const long o = (long)i * s + k;
+  const float x = __builtin_fmaf(dt, v[o], u[o]);
+  u[o] = x;
+  if(x != x) atomicOr(&diag->flags, 1u);
+}
+
+// Ghost rows, dim 1, all columns including dim-2 ghosts (src/bcond.cu:31-36,45-57).
+template<int NV>
+__global__ void bcond1_kernel(float *u, const long s, const int n1, const int n2, const int p1)
+{
+  const int w = (n2 + 2 * HALF) * NV;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;   // float within the row
+  const int g = blockIdx.y;                              // ghost row 0..5
+  if(k >= w) return;
+  const int lo = g < HALF;
+  const int kk = lo ? g : g - HALF;
+  const int dst = lo ? -(kk + 1) : n1 + kk;
+  const int src = lo ? (p1 ? n1 - (kk + 1) : kk) : (p1 ? kk : n1 - 1 - kk);
+  float *row0 = u - HALF * NV;
+  row0[(long)dst * s + k] = row0[(long)src * s + k];
+}
+
+// Ghost columns, dim 2, all rows including dim-1 ghosts (src/bcond.cu:38-43,59-75).
+template<int NV, int ODD>
+__global__ void bcond2_kernel(float *u, const long s, const int n1, const int n2, const int p2)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;   // (ghost col, var)
+  const int r = blockIdx.y - HALF;                       // row -3 .. n1+2
+  if(t >= 2 * HALF * NV) return;
+  const int q = t % NV, g = t / NV;
+  const int lo = g < HALF;
+  const int kk = lo ? g : g - HALF;
+  const int dst = lo ? -(kk + 1) : n2 + kk;
+  const int src = lo ? (p2 ? n2 - (kk + 1) : kk) : (p2 ? kk : n2 - 1 - kk);
+  float *row = u + (long)r * s;
+  const float x = row[(long)src * NV + q];
+  row[(long)dst * NV + q] = (!p2 && q == ODD) ? -x : x;
+}
+
+template<class M>
+__global__ __launch_bounds__(256)
+void euler_kernel(float *u, const long s, const Params P, const float dt)
+{
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if(j >= P.n2) return;
+  float *c = u + (long)i * s + (long)j * M::NV;
+  M::first_order(P, c, dt, i, j);
+}
+
+// CFL rates of u for a state no fused stage-2 epilogue has reduced (the
+// first step, or after an upload): columns across threads, rows strided
+// over gridDim.y, one block reduction and one atomic per block.
+template<class M>
+__global__ __launch_bounds__(256)
+void cfl_kernel(const float *u, const long s, const Params P, DeviceDiag *diag)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const float k1a = 1.5866979575f / P.Delta1, k1b = 1.5866979575f / P.Delta2;
+  float la = 0.0f, ld = 0.0f;
+  if(j < P.n2)
+    for(int i = blockIdx.y; i < P.n1; i += gridDim.y) {
+      float c[M::NV];
+      load_cell<M::NV>(u + (long)i * s + (long)j * M::NV, c);
+      la = fmaxf(la, M::lam_adv(P, c, i, j, k1a, k1b));
+      if constexpr(M::CELL_DIFF) ld = fmaxf(ld, M::lam_diff(P, c, i, j));
+    }
+  la = wave_max(la);
+  ld = wave_max(ld);
+  __shared__ float red[2][4];
+  const int w = threadIdx.x >> 6;
+  if((threadIdx.x & 63) == 0) { red[0][w] = la; red[1][w] = ld; }
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    for(int k = 1; k < 4; ++k) { la = fmaxf(la, red[0][k]); ld = fmaxf(ld, red[1][k]); }
+    atomicMax(&diag->lam_adv_bits, (unsigned long long)__double_as_longlong((double)la));
+    atomicMax(&diag->lam_diff_bits, (unsigned long long)__double_as_longlong((double)ld));
+  }
+}
+
+// FP64 sums per row (pass 1) and over rows in fixed order (pass 2).
+// out[0] = sum rho, out[1] = sum rho u1, out[2] = sum rho u2,
+// out[3] = sum rho Omg R^2 (disk; R = sphr sin theta), all times the cell area
+// in index space (the caller scales).
+template<class M>
+__global__ __launch_bounds__(256)
+void sums_row_kernel(const float *u, const long s, const Params P, double *partial)
+{
+  const int i = blockIdx.x;
+  double acc[4] = {0, 0, 0, 0};
+  for(int j = threadIdx.x; j < P.n2; j += blockDim.x) {
+    const float *c = u + (long)i * s + (long)j * M::NV;
+    const double rho = exp((double)c[0]);
+    if constexpr(M::ID == DISK) {
+      const double sphr = (double)P.par[2] * exp((i + 0.5) * (double)P.Delta1);
+      const double th = (j + 0.5) * (double)P.Delta2;
+      const double R = sphr * sin(th);
+      // volume element of the (log r, theta) grid: r^3 sin(theta) dlnr dtheta
+      const double dV = sphr * sphr * sphr * sin(th);
+      acc[0] += rho * dV;
+      acc[1] += rho * c[1] * dV;
+      acc[2] += rho * c[2] * dV;
+      acc[3] += rho * c[3] * R * R * dV;
+    } else {
+      acc[0] += rho;
+      acc[1] += rho * c[1];
+      acc[2] += rho * c[2];
+    }
+  }
+  __shared__ double red[4][256];
+#pragma unroll
+  for(int k = 0; k < 4; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for(int o = 128; o > 0; o >>= 1) {
+    if((int)threadIdx.x < o)
+#pragma unroll
+      for(int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if(threadIdx.x < 4) partial[(long)threadIdx.x * P.n1 + i] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256)
+void sums_final_kernel(const double *partial, const int n1, double *out)
+{
+  __shared__ double red[256];
+  for(int k = 0; k < 4; ++k) {
+    double a = 0;
+    for(int i = threadIdx.x; i < n1; i += blockDim.x) a += partial[(long)k * n1 + i];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for(int o = 128; o > 0; o >>= 1) {
+      if((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if(threadIdx.x == 0) out[k] = red[0];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template<class F> static hipError_t by_module(int module, F f)
+{
+  switch(module) {
+  case HYDRO:      return f(Hydro{});
+  case HYDRO_ISOT: return f(HydroIsot{});
+  case DISK:       return f(Disk{});
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_bcond(int module, float *u, long s, const Params &P, int p1, int p2, hipStream_t st)
+{
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    constexpr int NV = M::NV;
+    const int w = (P.n2 + 2 * HALF) * NV;
+    bcond1_kernel<NV><<<dim3((w + 255) / 256, 2 * HALF), 256, 0, st>>>(u, s, P.n1, P.n2, p1);
+    bcond2_kernel<NV, M::ODD><<<dim3(1, P.n1 + 2 * HALF), 64, 0, st>>>(u, s, P.n1, P.n2, p2);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_kick(int module, float *v, const float *u, long s, const Params &P, float beta,
+                       hipStream_t st)
+{
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    kick_kernel<M><<<dim3((P.n2 + 255) / 256, P.n1), 256, 0, st>>>(v, u, s, P, beta);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_drift(int module, float *u, const float *v, long s, const Params &P, float dt,
+                        DeviceDiag *diag, hipStream_t st)
+{
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    const int n = P.n2 * M::NV;
+    drift_kernel<M::NV><<<dim3((n + 255) / 256, P.n1), 256, 0, st>>>(u, v, s, P.n2, dt, diag);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_euler(int module, float *u, long s, const Params &P, float dt, hipStream_t st)
+{
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    if constexpr(!M::HAS_FIRST_ORDER) return hipSuccess;
+    euler_kernel<M><<<dim3((P.n2 + 255) / 256, P.n1), 256, 0, st>>>(u, s, P, dt);
+    return hipGetLastError();
+  });
+}
+
+template<class M, int STAGE>
+static hipError_t launch_stage(const float *uin, float *uout, float *v, long s, const Params &P,
+                               float beta, float gdt, const FusedShape &sh, int cfl,
+                               DeviceDiag *diag, hipStream_t st)
+{
+  const int nstrips = (P.n2 + STRIP - 1) / STRIP;
+  const int nchunks = (P.n1 + sh.tile_h - 1) / sh.tile_h;
+  const int ntiles = nstrips * nchunks;
+  switch(sh.waves) {
+#define FG2_W(WPB)                                                              \
+  case WPB:                                                                    \
+    substage_kernel<M, STAGE, WPB><<<(ntiles + WPB - 1) / WPB, 64 * WPB, 0, st>>>( \
+      uin, uout, v, s, P, beta, gdt, sh.tile_h, nstrips, ntiles, cfl, diag);    \
+    break;
+  FG2_W(1) FG2_W(2) FG2_W(4)
+#undef FG2_W
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_substage(int module, int stage, const float *uin, float *uout, float *v, long s,
+                           const Params &P, float beta, float gdt, const FusedShape &sh, int cfl,
+                           DeviceDiag *diag, hipStream_t st)
+{
+  if(sh.tile_h < 1) return hipErrorInvalidValue;
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    switch(stage) {
+    case 0: return launch_stage<M, 0>(uin, uout, v, s, P, beta, gdt, sh, cfl, diag, st);
+    case 1: return launch_stage<M, 1>(uin, uout, v, s, P, beta, gdt, sh, cfl, diag, st);
+    case 2: return launch_stage<M, 2>(uin, uout, v, s, P, beta, gdt, sh, cfl, diag, st);
+    }
+    return hipErrorInvalidValue;
+  });
+}
+
+hipError_t launch_cfl(int module, const float *u, long s, const Params &P, DeviceDiag *diag,
+                      hipStream_t st)
+{
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    const int gx = (P.n2 + 255) / 256;
+    const int gy = std::max(1, std::min(P.n1, 2048 / gx));
+    cfl_kernel<M><<<dim3(gx, gy), 256, 0, st>>>(u, s, P, diag);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_sums(int module, const float *u, long s, const Params &P, double *partial,
+                       double *out, hipStream_t st)
+{
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    sums_row_kernel<M><<<P.n1, 256, 0, st>>>(u, s, P, partial);
+    sums_final_kernel<<<1, 256, 0, st>>>(partial, P.n1, out);
+    return hipGetLastError();
+  });
+}
+
+template<class M, int STAGE>
+static hipError_t launch_batch_stage(const BatchRun *runs, int nruns, int total, int waves,
+                                     int grid, int *counter, int cfl, hipStream_t st)
+{
+  switch(waves) {
+#define FG2_W(WPB)                                                                 \
+  case WPB:                                                                       \
+    batch_kernel<M, STAGE, WPB><<<grid, 64 * WPB, 0, st>>>(runs, nruns, total, counter, cfl); \
+    break;
+  FG2_W(1) FG2_W(2) FG2_W(4)
+#undef FG2_W
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_substage(int module, int stage, const BatchRun *runs, int nruns, int total,
+                                 int waves, int grid, int *counter, int cfl, hipStream_t st)
+{
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    switch(stage) {
+    case 0: return launch_batch_stage<M, 0>(runs, nruns, total, waves, grid, counter, cfl, st);
+    case 1: return launch_batch_stage<M, 1>(runs, nruns, total, waves, grid, counter, cfl, st);
+    case 2: return launch_batch_stage<M, 2>(runs, nruns, total, waves, grid, counter, cfl, st);
+    }
+    return hipErrorInvalidValue;
+  });
+}
+
+} // namespace fg2

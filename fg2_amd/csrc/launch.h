@@ -1,0 +1,56 @@
+// launch.h -- host-side launchers of the HIP kernels (kernels.hip), called
+// by the C-ABI driver (capi.hip).  All launches are asynchronous on the given
+// stream and return the hipError_t of the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "fg2_layout.h"
+
+namespace fg2 {
+
+// Plain path: one kernel per reference operation, ghost zones in memory.
+hipError_t launch_bcond(int module, float *u, long s, const Params &P, int p1, int p2,
+                        hipStream_t st);                                   // src/bcond.cu:77-86
+hipError_t launch_kick(int module, float *v, const float *u, long s, const Params &P,
+                       float beta, hipStream_t st);                        // src/kick.cu:108-114
+hipError_t launch_drift(int module, float *u, const float *v, long s, const Params &P,
+                        float dt, DeviceDiag *diag, hipStream_t st);       // src/drift.cu:34-39
+hipError_t launch_euler(int module, float *u, long s, const Params &P, float dt,
+                        hipStream_t st);                                   // src/kick.cu:128-138
+
+// Fused path: one launch per RK substage.  Reads u_in (boundaries by index
+// mapping, ghost zones unused) and v, writes v (stages 0 and 1) and u_out.
+// Stage 2 optionally reduces the CFL rates of u_out into diag.
+struct FusedShape {
+  int tile_h;      // rows marched by one wavefront
+  int waves;       // wavefronts per workgroup (each owns a 64-column strip tile)
+};
+hipError_t launch_substage(int module, int stage, const float *uin, float *uout, float *v,
+                           long s, const Params &P, float beta, float gdt,
+                           const FusedShape &shape, int cfl, DeviceDiag *diag,
+                           hipStream_t st);
+
+// CFL rates (FP64 max over cells of the advective and diffusive rates)
+// of a state, folded into diag->lam_*_bits.
+hipError_t launch_cfl(int module, const float *u, long s, const Params &P, DeviceDiag *diag,
+                      hipStream_t st);
+
+// Diagnostics: FP64 sums of density, momentum and (disk) angular momentum
+// over the interior, written to out[0..3] (deterministic two-pass reduction;
+// partial must hold 4 * n1 doubles).
+hipError_t launch_sums(int module, const float *u, long s, const Params &P, double *partial,
+                       double *out, hipStream_t st);
+
+// Batched path: many independent runs in one persistent launch.
+struct BatchRun {
+  const float *uin; float *uout; float *v;
+  long s;
+  Params P;
+  int tile_h, ntiles, nstrips, tile_begin;  // this run's tiles are [tile_begin, tile_begin+ntiles)
+  float beta, gdt;
+  DeviceDiag *diag;
+};
+hipError_t launch_batch_substage(int module, int stage, const BatchRun *runs_dev, int nruns,
+                                 int total_tiles, int waves, int grid, int *counter,
+                                 int cfl, hipStream_t st);
+
+} // namespace fg2

@@ -1,0 +1,96 @@
+"""Seeded synthetic initial states for the benchmark and test workloads.
+
+Host-side set-up (numpy), not part of the time-step path.  Coordinates are
+cell centred, x = l1 (i + 1/2) / n1 along dim 1 and y = l2 (j + 1/2) / n2
+along dim 2 (src/init.cpp:31-37).  The reference draws its random
+perturbations with libc rand() (src/hydro/init.h:68-71,
+src/hydro-isot/init.h:23-24); these generators use numpy's PCG64 with an
+explicit seed so every workload is reproducible.
+"""
+import numpy as np
+
+GAMMA = 5.0 / 3.0
+
+
+def gaussian_random_field(n1, n2, kmax, rms, seed, ncomp=1):
+    """Real random field(s) on a periodic n1 x n2 grid built from every
+    integer wavevector 0 < |k| <= kmax with independent complex normal
+    amplitudes, scaled so the root-mean-square (over cells, of the vector
+    norm when ncomp > 1) equals rms."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n1, 1.0 / n1)[:, None]
+    k2 = np.fft.fftfreq(n2, 1.0 / n2)[None, :]
+    mask = ((k1 ** 2 + k2 ** 2) <= kmax ** 2) & ((k1 != 0) | (k2 != 0))
+    out = []
+    for _ in range(ncomp):
+        spec = (rng.standard_normal((n1, n2)) + 1j * rng.standard_normal((n1, n2))) * mask
+        out.append(np.real(np.fft.ifft2(spec)))
+    f = np.stack(out, axis=-1)
+    norm = np.sqrt(np.mean(np.sum(f ** 2, axis=-1)))
+    return f * (rms / norm if norm > 0 else 0.0)
+
+
+def hydro_random(n1, n2, seed=1, kmax=4, rms=0.01, gamma=GAMMA):
+    """hydro: rho = 1, p = 1/gamma (so c_s = 1), velocity a Gaussian random
+    field with |k| <= kmax and rms |u| = rms * c_s.  Returns (n1, n2, 4)."""
+    u = gaussian_random_field(n1, n2, kmax, rms, seed, ncomp=2)
+    x = np.zeros((n1, n2, 4))
+    x[..., 1:3] = u
+    x[..., 3] = np.log((1.0 / gamma) / (gamma - 1.0))   # e = p / (rho (gamma - 1))
+    return x
+
+
+def hydro_kh(n1, n2, seed=2, amp=0.01, gamma=GAMMA, l1=1.0, l2=1.0):
+    """Kelvin-Helmholtz shear layer on [-l/2, l/2]^2 (as src/hydro/init.h:62-63):
+    rho = 2, u1 = +0.5 for |y| < 0.25, rho = 1, u1 = -0.5 elsewhere, p = 2.5,
+    u2 = amp * N(0, 1) white noise.  Returns (n1, n2, 4)."""
+    rng = np.random.default_rng(seed)
+    y = l2 * (np.arange(n2) + 0.5) / n2 - 0.5 * l2
+    inner = np.abs(y) < 0.25
+    rho = np.where(inner, 2.0, 1.0)[None, :].repeat(n1, 0)
+    x = np.zeros((n1, n2, 4))
+    x[..., 0] = np.log(rho)
+    x[..., 1] = np.where(inner, 0.5, -0.5)[None, :]
+    x[..., 2] = amp * rng.standard_normal((n1, n2))
+    x[..., 3] = np.log(2.5 / rho / (gamma - 1.0))
+    return x
+
+
+def isot_density(n1, n2, seed=3, kmax=8, rms=0.05):
+    """hydro-isot: lnd = ln(1 + delta), delta a Gaussian random field with
+    |k| <= kmax and rms 0.05, u = 0.  Returns (n1, n2, 3)."""
+    d = gaussian_random_field(n1, n2, kmax, rms, seed)[..., 0]
+    x = np.zeros((n1, n2, 3))
+    x[..., 0] = np.log1p(d)
+    return x
+
+
+def disk_kepler(n1, n2, seed=4, l1=None, rin=3.0, gm=1.0, h_over_r=0.05,
+                noise=0.01, floor=1e-4, gamma=GAMMA, lner=-30.0):
+    """disk: Keplerian disc around a central mass on the (log r, theta) grid
+    of src/disk/config.h:21-34.  Sigma ~ R^-1 (rho_mid ~ R^-2) with a
+    Gaussian vertical profile of aspect ratio h_over_r, a density floor,
+    Omega = sqrt(GM / R^3), isothermal sound speed h_over_r * v_K, radial and
+    vertical velocity 0, radiation energy at a floor, and 1% seeded density
+    noise.  Returns (n1, n2, 6)."""
+    rng = np.random.default_rng(seed)
+    l2 = np.pi
+    if l1 is None:
+        hd2 = 0.5 * l2 / n2
+        l1 = 2.0 * np.arcsinh(hd2) * n1
+    lnr = l1 * (np.arange(n1) + 0.5) / n1
+    th = l2 * (np.arange(n2) + 0.5) / n2
+    sphr = rin * np.exp(lnr)[:, None]
+    R = sphr * np.sin(th)[None, :]
+    z = sphr * np.cos(th)[None, :]
+    H = h_over_r * R
+    rho = R ** -2.0 * np.exp(-0.5 * (z / H) ** 2)
+    rho = np.maximum(rho, floor) * (1.0 + noise * rng.standard_normal((n1, n2)))
+    vk = np.sqrt(gm / R)
+    cs2 = (h_over_r * vk) ** 2
+    x = np.zeros((n1, n2, 6))
+    x[..., 0] = np.log(rho)
+    x[..., 3] = np.sqrt(gm / R ** 3)
+    x[..., 4] = np.log(cs2 / (gamma - 1.0))      # (gamma - 1) e = c_s^2
+    x[..., 5] = lner
+    return x

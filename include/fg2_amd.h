@@ -1,0 +1,182 @@
+/* fg2_amd.h -- C ABI of the MI355X (gfx950) implementation of fg2's
+ * time-step path: kick, drift, boundary fill, first_order, RK3 step, time
+ * step, the parameter registry, and batched execution of many runs.
+ *
+ * Every entry point returns an int status (FG2_OK = 0) unless stated.  All
+ * device work of a run is ordered on the run's own HIP stream; fg2_step and
+ * fg2_solve synchronise that stream once per step, as the reference's step()
+ * does (src/step.cpp:36).  Pointers named u, v are device pointers to
+ * interior cell (0,0) of an array in the run's layout (fg2_layout): NVAR
+ * floats per cell (AoS), row stride `stride` floats, 3 ghost cells/rows on
+ * every side at negative / past-the-end offsets -- the convention of the
+ * reference's global::u / global::v (src/setup.cpp:60-72).
+ *
+ * Not reentrant per run: calls on one run must not overlap.  Different runs
+ * may be driven from different host threads.
+ */
+#ifndef FG2_AMD_H
+#define FG2_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fg2_run fg2_run;
+typedef struct fg2_batch fg2_batch;
+
+enum {
+  FG2_OK        = 0,
+  FG2_ERR_NAN   = 1,  /* the state holds a NaN after a step; the reference returns
+                         cudaErrorInvalidValue here (src/step.cpp:38-40)            */
+  FG2_ERR_HIP   = 2,  /* a HIP runtime error (src/step.cpp:42 returns the CUDA error) */
+  FG2_ERR_SHAPE = 3,  /* grid or tile shape not supported                            */
+  FG2_ERR_ARG   = 4,  /* bad argument: unknown module or parameter, NULL pointer     */
+  FG2_ERR_NOMEM = 5   /* device allocation failed (setup() returns 0, src/setup.cpp:62-72) */
+};
+
+/* Step implementation used by fg2_step / fg2_solve. */
+enum {
+  FG2_PATH_FUSED = 0, /* one fused kick+drift launch per RK substage (default)        */
+  FG2_PATH_PLAIN = 1  /* bcond, kick, drift launches per substage (src/step.cpp:30-35) */
+};
+
+/* Time-step rule used by fg2_solve. */
+enum {
+  FG2_DT_CLOSED = 0,  /* c / (n1/l1 + n2/l2), src/getdt.cpp:26 (default)               */
+  FG2_DT_CFL    = 1   /* FP64 min over cells of the advective and diffusive limits    */
+};
+
+/* Text of the last error on this host thread (never NULL). */
+const char *fg2_last_error(void);
+
+/* ---- set-up: src/setup.cpp:32-87, src/config.cpp:24-41, src/para.cpp:25-40 ---- */
+
+/* Allocate a run of `module` ("hydro", "hydro-isot" or "disk") on n1 x n2
+ * cells on the current HIP device; replaces setup(n1, n2) (src/fg2.h:74,
+ * src/setup.cpp:32).  *bytes (may be NULL) receives the device bytes
+ * allocated.  Defaults as src/global.cpp:28-29: l1 = l2 = 1, c = 0.5,
+ * periodic in both dimensions; module parameters at their reference values.
+ * n1, n2 >= 3. */
+int fg2_setup(fg2_run **run, const char *module, int n1, int n2, size_t *bytes);
+
+/* Free a run (the reference frees at exit, src/setup.cpp:24-30). */
+void fg2_free(fg2_run *run);
+
+/* Geometry and boundary flags (global::l1, l2, c, p1, p2; src/fg2.h:54-56). */
+int fg2_set_geometry(fg2_run *run, double l1, double l2, int p1, int p2, double c);
+/* out[7] = l1, l2, p1, p2, c, Delta1, Delta2 */
+int fg2_get_geometry(const fg2_run *run, double *out);
+
+/* Apply the module's settings() and compute Delta = l / n; replaces config()
+ * (src/fg2.h:75, src/config.cpp:24-41). */
+int fg2_config(fg2_run *run);
+
+/* "name=val" (leading dashes ignored) -> sets module parameter para_<name>;
+ * returns "name = val" in a buffer owned by the run, or NULL if the module
+ * has no such parameter.  Replaces para() (src/fg2.h:73, src/para.cpp:25-40). */
+const char *fg2_para(fg2_run *run, const char *assignment);
+int fg2_get_param(const fg2_run *run, const char *name, double *value);
+/* Seed of the hydro-isot first_order random phases (libc rand() in
+ * src/hydro-isot/Euler.cuh:39-44). */
+int fg2_seed(fg2_run *run, unsigned long long seed);
+
+/* Layout: NVAR, row stride in floats, and the interior pointers of the
+ * current state u and the swap array v (global::u, global::v). */
+int fg2_layout(const fg2_run *run, int *nvar, long *stride);
+float *fg2_u(fg2_run *run);
+float *fg2_v(fg2_run *run);
+/* The run's HIP stream (hipStream_t). */
+void *fg2_stream(fg2_run *run);
+
+/* Host <-> device copies of the interior, n1*n2*NVAR floats in .raw order
+ * (no ghosts); replace the cudaMemcpy2D of src/init.cpp:39-41 and
+ * src/io.cpp:44-45,70-72. */
+int fg2_upload(fg2_run *run, const float *host);
+int fg2_download(fg2_run *run, float *host);
+int fg2_upload_v(fg2_run *run, const float *host);
+int fg2_download_v(fg2_run *run, float *host);
+
+/* ---- device operations, 1:1 with src/fg2.h:81-84 (asynchronous) ---- */
+
+/* Point-wise first_order pre-step; replaces Euler(u, dt) (src/kick.cu:128-138). */
+int fg2_euler(fg2_run *run, float *u, float dt);
+/* Ghost fill, dim 1 then dim 2; replaces bcond(u, p1, p2) (src/bcond.cu:77-86). */
+int fg2_bcond(fg2_run *run, float *u, int p1, int p2);
+/* v <- F(u) + beta v on the interior (u's ghosts must be filled); replaces
+ * kick(v, u, t, beta) (src/kick.cu:108-114).  t is unused, as in the reference. */
+int fg2_kick(fg2_run *run, float *v, const float *u, float t, float beta);
+/* u <- u + dt v on the interior, sticky NaN flag; replaces drift(u, v, dt)
+ * (src/drift.cu:34-39). */
+int fg2_drift(fg2_run *run, float *u, const float *v, float dt);
+
+/* ---- integration: src/fg2.h:77-79 ---- */
+
+/* Closed-form time step c / (n1/l1 + n2/l2); replaces getdt()
+ * (src/getdt.cpp:22-29).  Returns the step (no status). */
+double fg2_getdt(const fg2_run *run);
+/* CFL time step of the current state: c * min(sqrt(3)/lambda_adv,
+ * 2.5127/lambda_diff), the rates reduced over all cells in FP64 on the
+ * device.  New: the reference leaves this as a TODO (src/getdt.cpp:26-28). */
+int fg2_getdt_cfl(fg2_run *run, double *dt);
+
+/* Path and time-step rule (FG2_PATH_*, FG2_DT_*). */
+int fg2_set_path(fg2_run *run, int path);
+int fg2_set_dt_mode(fg2_run *run, int mode);
+/* Fused-kernel tile: rows per wavefront and wavefronts per workgroup
+ * (1, 2 or 4).  tile_h = 0 restores the automatic choice. */
+int fg2_set_tile(fg2_run *run, int tile_h, int waves);
+int fg2_get_tile(const fg2_run *run, int *tile_h, int *waves);
+
+/* One low-storage RK3 step of size dt at time t: first_order, then three
+ * substages; synchronises the run's stream and checks the NaN flag.
+ * Replaces step(t, dt) (src/step.cpp:22-43): returns FG2_OK, FG2_ERR_NAN
+ * or FG2_ERR_HIP. */
+int fg2_step(fg2_run *run, double t, double dt);
+/* Same as fg2_step without the synchronisation and the NaN check, so that
+ * many steps can be queued or captured in a HIP graph; check the flag
+ * later with fg2_check. */
+int fg2_step_async(fg2_run *run, double t, double dt);
+/* Synchronise the run's stream; FG2_ERR_NAN if any step since the last
+ * check produced a NaN (the flag is then cleared). */
+int fg2_check(fg2_run *run);
+
+/* Advance from *t to t_final with dt = min(t_final - t, getdt) per step, as
+ * the inner loop of solve() (src/solve.cpp:37-49); *t and *steps (may be
+ * NULL) receive the final time and the number of steps.  On a NaN the state
+ * of the failing step is kept and FG2_ERR_NAN is returned (the reference
+ * dumps it, src/solve.cpp:41-46). */
+int fg2_solve(fg2_run *run, double *t, double t_final, int *steps);
+
+/* FP64 diagnostics of the current state: out[0] = sum rho, out[1..2] =
+ * sum rho u1, sum rho u2 (disk: rho ur, rho uz), out[3] = sum rho Omg R^2
+ * (disk), each times the cell volume element (index-space for the
+ * Cartesian modules, r^3 sin(theta) for disk). */
+int fg2_sums(fg2_run *run, double *out);
+
+/* Perf model of the reference (src/config.cpp:30-36): flop and bits per
+ * step; and the algorithmic HBM bytes per cell per step of the fused path,
+ * 10 NVAR sizeof(float). */
+int fg2_perf_model(const fg2_run *run, double *flops, double *bits, double *alg_bytes_per_cell);
+
+/* ---- batched execution: many independent runs advanced together ---- */
+
+/* A batch of runs of one module.  Each fg2_batch_step advances every run of
+ * the batch by one RK3 step of its own dt (dts[k]) with three persistent
+ * launches for the whole batch.  The runs keep their own state; a run
+ * gives the same bits alone (fg2_step, fused path) and in a batch. */
+int fg2_batch_create(fg2_batch **batch, fg2_run *const *runs, int nruns, int waves, int tile_h);
+void fg2_batch_free(fg2_batch *batch);
+int fg2_batch_step_async(fg2_batch *batch, const double *dts);
+int fg2_batch_step(fg2_batch *batch, const double *dts);
+/* Capture `steps` batch steps with fixed per-run dts into a HIP graph and
+ * replay it `replays` times. */
+int fg2_batch_graph_run(fg2_batch *batch, const double *dts, int steps, int replays);
+void *fg2_batch_stream(fg2_batch *batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FG2_AMD_H */

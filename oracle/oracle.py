@@ -1,0 +1,213 @@
+"""ctypes wrapper of the host oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module; it is the checker and the
+CPU baseline, never the thing measured or shipped.  The product package
+(fg2_amd) does not import it.
+
+The oracle restates the reference's time-step path on the host, templated on
+the storage type: ``fp64=True`` is the FP64 oracle, ``fp64=False`` the FP32 CPU
+baseline.  Each function of oracle.cpp cites the reference file:line it follows.
+
+Pinning (which test fixes each function against something other than itself):
+
+  operators D1 D2 D11 D22 D12 (src/deriv.h:41-63)
+      polynomial exactness to degree 6/7, Fourier modified wavenumbers in
+      closed form, D12 at (pi, pi) = 0                  tests/test_oracle_operators.py
+  bcond (src/bcond.cu:22-86)
+      brute force against numpy roll / mirror, corners    tests/test_oracle_bcond.py
+  drift (src/drift.cu:23-32), kick's beta*v + F order (src/kick.cu:41-57)
+      element-wise brute force                            tests/test_oracle_step.py
+  RK3 step (src/step.cpp:22-43)
+      linear-mode amplification 1 + z + z^2/2 + z^3/6     tests/test_oracle_step.py
+  hydro, hydro-isot RHS (src/hydro/eqns.cuh, src/hydro-isot/eqns.cuh)
+      uniform state -> 0, transpose symmetry, manufactured smooth fields
+      against sympy's continuous RHS (committed fixtures), sound speed,
+      viscous and thermal decay rates, Kolmogorov forcing  tests/test_oracle_eqns.py
+  disk RHS (src/disk/eqns.cuh)
+      uniform state -> gravity only, Keplerian balance of the radial
+      equation, manufactured smooth fields against sympy's continuous RHS
+      (chain rule from (log r, theta) to (R, z))           tests/test_oracle_eqns.py
+  first_order / Euler (src/*/Euler.cuh)
+      hydro: identity.  disk: the quartic root x^4 + x - x0 = 0 by
+      substitution.  hydro-isot forcing: parity unpinned beyond the
+      zero-forcing identity (its random phases use libc rand() in the
+      reference, replaced here by SplitMix64).            tests/test_oracle_step.py
+  getdt closed form (src/getdt.cpp:26)
+      closed form values                                  tests/test_oracle_step.py
+  CFL time step (new; src/getdt.cpp:26-28 is a TODO)
+      stability constants: |R(i sqrt3)| = 1, |R(-2.5127)| = 1, max of the
+      D1 modified wavenumber                              tests/test_oracle_step.py
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+MODULES = {"hydro": 0, "hydro-isot": 1, "disk": 2}
+NVAR = {"hydro": 4, "hydro-isot": 3, "disk": 6}
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        DP = ctypes.POINTER(ctypes.c_double)
+        sig = {
+            "orc_create": (P, [I, I, I, I]),
+            "orc_destroy": (None, [P]),
+            "orc_nvar": (I, [P]),
+            "orc_set_param": (I, [P, ctypes.c_char_p, D]),
+            "orc_get_param": (D, [P, ctypes.c_char_p]),
+            "orc_set_geometry": (None, [P, D, D, I, I, D]),
+            "orc_get_geometry": (None, [P, DP]),
+            "orc_config": (None, [P]),
+            "orc_seed": (None, [P, ctypes.c_ulonglong]),
+            "orc_get_rand": (None, [P, DP]),
+            "orc_upload": (None, [P, DP]),
+            "orc_download": (None, [P, DP]),
+            "orc_download_ghosted": (None, [P, DP]),
+            "orc_upload_v": (None, [P, DP]),
+            "orc_download_v": (None, [P, DP]),
+            "orc_bcond": (None, [P]),
+            "orc_kick": (None, [P, D]),
+            "orc_drift": (None, [P, D]),
+            "orc_euler": (None, [P, D]),
+            "orc_rhs": (None, [P, DP]),
+            "orc_step": (I, [P, D, D]),
+            "orc_run": (I, [P, D, D, I]),
+            "orc_getdt": (D, [P]),
+            "orc_getdt_cfl": (D, [P]),
+            "orc_nan_flag": (I, [P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """One simulation on the host: state u, swap v, geometry and parameters."""
+
+    def __init__(self, module, n1, n2, fp64=True, params=None, geometry=None,
+                 config=True, seed=0):
+        self.module = module
+        self.n1, self.n2 = n1, n2
+        self.nv = NVAR[module]
+        self.h = lib().orc_create(MODULES[module], n1, n2, 1 if fp64 else 0)
+        if not self.h:
+            raise ValueError("bad oracle shape")
+        for k, v in (params or {}).items():
+            self.set_param(k, v)
+        if geometry is not None:
+            self.set_geometry(**geometry)
+        if config:
+            self.config()
+        lib().orc_seed(self.h, seed)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    @property
+    def shape(self):
+        return (self.n1, self.n2, self.nv)
+
+    def set_param(self, name, val):
+        if lib().orc_set_param(self.h, name.encode(), float(val)) != 0:
+            raise KeyError(name)
+
+    def get_param(self, name):
+        return lib().orc_get_param(self.h, name.encode())
+
+    def set_geometry(self, l1=1.0, l2=1.0, p1=1, p2=1, c=0.5):
+        lib().orc_set_geometry(self.h, l1, l2, int(p1), int(p2), c)
+
+    def geometry(self):
+        out = np.zeros(7)
+        lib().orc_get_geometry(self.h, _dp(out))
+        keys = ("l1", "l2", "p1", "p2", "c", "Delta1", "Delta2")
+        return dict(zip(keys, out.tolist()))
+
+    def config(self):
+        lib().orc_config(self.h)
+
+    def rand(self):
+        out = np.zeros(2)
+        lib().orc_get_rand(self.h, _dp(out))
+        return out
+
+    def upload(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.shape)
+        lib().orc_upload(self.h, _dp(x))
+
+    def download(self):
+        x = np.empty(self.shape)
+        lib().orc_download(self.h, _dp(x))
+        return x
+
+    def upload_v(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.shape)
+        lib().orc_upload_v(self.h, _dp(x))
+
+    def download_v(self):
+        x = np.empty(self.shape)
+        lib().orc_download_v(self.h, _dp(x))
+        return x
+
+    def download_ghosted(self):
+        x = np.empty((self.n1 + 6, self.n2 + 6, self.nv))
+        lib().orc_download_ghosted(self.h, _dp(x))
+        return x
+
+    def bcond(self):
+        lib().orc_bcond(self.h)
+
+    def kick(self, beta):
+        lib().orc_kick(self.h, beta)
+
+    def drift(self, dt):
+        lib().orc_drift(self.h, dt)
+
+    def euler(self, dt):
+        lib().orc_euler(self.h, dt)
+
+    def rhs(self):
+        out = np.empty(self.shape)
+        lib().orc_rhs(self.h, _dp(out))
+        return out
+
+    def step(self, t, dt):
+        return lib().orc_step(self.h, t, dt)
+
+    def run(self, t, dt, n):
+        return lib().orc_run(self.h, t, dt, n)
+
+    def getdt(self):
+        return lib().orc_getdt(self.h)
+
+    def getdt_cfl(self):
+        return lib().orc_getdt_cfl(self.h)
+
+    def nan_flag(self):
+        return lib().orc_nan_flag(self.h)

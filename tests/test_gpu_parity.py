@@ -1,0 +1,172 @@
+"""GPU parity: the HIP path (called through the C ABI) against the FP64 host
+oracle, and the fused path against the plain per-operation path.
+
+Bars:
+  - fused vs plain, any tile shape, alone vs batched: bit-identical (the
+    kernels share one per-cell instruction sequence);
+  - FP32 GPU vs FP64 oracle after N steps: per-field relative L2 and Linf
+    below the tolerances stated in each test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2, rel_linf, state
+
+pytestmark = pytest.mark.gpu
+
+MODULES = ["hydro", "hydro-isot", "disk"]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _run(fg2, module, n1, n2, x, path, geometry=None, tile=None, params=None):
+    r = fg2.Run(module, n1, n2, geometry=geometry, params=params, path=path, tile=tile)
+    r.upload(x)
+    return r
+
+
+@pytest.mark.parametrize("module", MODULES)
+@pytest.mark.parametrize("periodic", [1, 0])
+def test_fused_equals_plain(fg2, module, periodic):
+    n1, n2 = 72, 80          # 80 columns: one full and one partial 64-column strip
+    _say(f"case fused==plain {module} periodic={periodic} {n1}x{n2}")
+    x = state(module, n1, n2)
+    geo = None if module == "disk" else dict(l1=1.0, l2=1.0, p1=periodic, p2=periodic, c=0.5)
+    a = _run(fg2, module, n1, n2, x, fg2.PATH_PLAIN, geo)
+    b = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, geo, tile=(16, 1))
+    dt = 0.25 * a.getdt()
+    for k in range(3):
+        assert a.step(k * dt, dt) == 0
+        assert b.step(k * dt, dt) == 0
+    ya, yb = a.download(), b.download()
+    assert np.all(np.isfinite(ya))
+    np.testing.assert_array_equal(ya, yb)
+
+
+@pytest.mark.parametrize("module", MODULES)
+def test_tile_shapes_bit_identical(fg2, module):
+    n1, n2 = 100, 150
+    _say(f"case tile shapes {module} {n1}x{n2}")
+    x = state(module, n1, n2, seed=5)
+    ref = None
+    for tile in [(8, 1), (13, 2), (64, 4), (100, 1), (256, 1)]:
+        r = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, tile=tile)
+        dt = 0.25 * r.getdt()
+        assert r.run(0.0, dt, 2) == 0
+        y = r.download()
+        if ref is None:
+            ref = y
+        else:
+            np.testing.assert_array_equal(y, ref)
+
+
+@pytest.mark.parametrize("module,nsteps,tol", [
+    ("hydro", 20, 2e-6), ("hydro-isot", 20, 2e-6), ("disk", 5, 1e-5)])
+@pytest.mark.parametrize("periodic", [1, 0])
+def test_vs_oracle_small(fg2, module, nsteps, tol, periodic):
+    from oracle.oracle import Oracle
+    n1, n2 = 48, 70
+    _say(f"case vs oracle {module} periodic={periodic} {n1}x{n2} steps={nsteps}")
+    x = state(module, n1, n2, seed=7)
+    geo = None if module == "disk" else dict(l1=1.0, l2=1.3, p1=periodic, p2=periodic, c=0.5)
+    r = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, geo)
+    o = Oracle(module, n1, n2, geometry=geo)
+    o.upload(x.astype(np.float32).astype(np.float64))
+    dt = 0.25 * r.getdt()
+    assert r.run(0.0, dt, nsteps) == 0
+    assert o.run(0.0, dt, nsteps) == 0
+    y, z = r.download(), o.download()
+    e2, ei = rel_l2(y, z), rel_linf(y, z)
+    _say("rel L2", e2, "rel Linf", ei)
+    assert np.all(e2 < tol), e2
+    assert np.all(ei < 10 * tol), ei
+
+
+def test_config1_hydro_256_100_steps(fg2):
+    """hydro 256^2 periodic, gamma=5/3, rho=1, p=1/gamma, Gaussian random
+    velocity |k|<=4 with rms 0.01 c_s, closed-form dt, 100 steps: FP32 GPU
+    within 1e-5 relative L2 of the FP64 oracle on every stored field."""
+    from oracle.oracle import Oracle
+    from fg2_amd import ics
+    n = 256
+    _say("case config1 hydro 256^2 x 100")
+    x = ics.hydro_random(n, n, seed=1)
+    r = _run(fg2, "hydro", n, n, x, fg2.PATH_FUSED)
+    o = Oracle("hydro", n, n)
+    o.upload(x.astype(np.float32).astype(np.float64))
+    dt = r.getdt()
+    assert abs(dt - 0.5 / 512) < 1e-18
+    assert r.run(0.0, dt, 100) == 0
+    assert o.run(0.0, dt, 100) == 0
+    y, z = r.download(), o.download()
+    e2 = rel_l2(y, z)
+    _say("config1 rel L2 per field", e2, "Linf", rel_linf(y, z))
+    assert np.all(e2 < 1e-5), e2
+    # physical fields
+    rho_err = np.sqrt(np.mean((np.exp(y[..., 0]) - np.exp(z[..., 0])) ** 2))
+    assert rho_err < 1e-6
+
+
+def test_nan_detection(fg2):
+    """A NaN in the state makes step() return the NaN code (src/step.cpp:38-40)."""
+    n = 64
+    x = state("hydro", n, n)
+    x[10, 20, 1] = np.nan
+    for path in (fg2.PATH_FUSED, fg2.PATH_PLAIN):
+        r = _run(fg2, "hydro", n, n, x, path)
+        assert r.step(0.0, 1e-4) == fg2.ERR_NAN
+        # the flag is sticky until read, then cleared
+        r.upload(state("hydro", n, n))
+        assert r.step(0.0, 1e-4) == 0
+
+
+def test_cfl_vs_oracle(fg2):
+    from oracle.oracle import Oracle
+    for module in MODULES:
+        n1, n2 = 40, 56
+        x = state(module, n1, n2, seed=3)
+        r = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED)
+        o = Oracle(module, n1, n2)
+        o.upload(x.astype(np.float32).astype(np.float64))
+        a, b = r.getdt_cfl(), o.getdt_cfl()
+        _say(module, "cfl", a, b)
+        assert abs(a - b) <= 1e-5 * b
+        # the fused stage-2 epilogue reduces the same rates for the next step
+        r.set_dt_mode(fg2.DT_CFL)
+        assert r.step(0.0, 0.1 * a) == 0
+        c = r.getdt_cfl()
+        o.run(0.0, 0.1 * a, 1)
+        assert abs(c - o.getdt_cfl()) <= 1e-5 * c
+
+
+def test_batch_equals_alone(fg2):
+    """Runs of uneven sizes give the same bits alone and in a batch."""
+    sizes = [(64, 96), (128, 64), (96, 160), (70, 70)]
+    xs = [state("hydro-isot", a, b, seed=10 + k) for k, (a, b) in enumerate(sizes)]
+    alone = []
+    for (a, b), x in zip(sizes, xs):
+        r = _run(fg2, "hydro-isot", a, b, x, fg2.PATH_FUSED, tile=(32, 1))
+        dt = 0.2 * r.getdt()
+        assert r.run(0.0, dt, 3) == 0
+        alone.append(r.download())
+    runs = [_run(fg2, "hydro-isot", a, b, x, fg2.PATH_FUSED) for (a, b), x in zip(sizes, xs)]
+    batch = fg2.Batch(runs, waves=1, tile_h=32)
+    dts = [0.2 * r.getdt() for r in runs]
+    for _ in range(3):
+        assert batch.step(dts) == 0
+    for r, y in zip(runs, alone):
+        np.testing.assert_array_equal(r.download(), y)
+    # graph replay of the same two steps twice == 4 eager steps
+    runs2 = [_run(fg2, "hydro-isot", a, b, x, fg2.PATH_FUSED) for (a, b), x in zip(sizes, xs)]
+    b2 = fg2.Batch(runs2, waves=2, tile_h=32)
+    b2.graph_run(dts, steps=2, replays=2)
+    runs3 = [_run(fg2, "hydro-isot", a, b, x, fg2.PATH_FUSED, tile=(32, 1)) for (a, b), x in zip(sizes, xs)]
+    for r, d in zip(runs3, dts):
+        assert r.run(0.0, d, 4) == 0
+    for r, q in zip(runs2, runs3):
+        np.testing.assert_array_equal(r.download(), q.download())

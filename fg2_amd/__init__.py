@@ -13,8 +13,8 @@ simulation in `namespace global`; here a `Run` holds it:
     Euler/bcond/kick/drift  -> Run.euler/bcond/kick/drift src/fg2.h:81-84
 
 All arithmetic runs in the HIP library; this module only moves host arrays
-and calls the C ABI.  Importing it raises if the library is not built:
-there is no CPU fallback.
+and calls the C ABI.  The library is opened on first use, which raises if
+it is not built: there is no CPU fallback.
 """
 import ctypes
 import os
@@ -85,6 +85,7 @@ def _load():
         "fg2_get_tile": (I, [P, IP, IP]),
         "fg2_step": (I, [P, D, D]),
         "fg2_step_async": (I, [P, D, D]),
+        "fg2_substage": (I, [P, I, D]),
         "fg2_check": (I, [P]),
         "fg2_solve": (I, [P, DP, D, IP]),
         "fg2_sums": (I, [P, DP]),
@@ -103,10 +104,20 @@ def _load():
     return L
 
 
-lib = _load()
+class _LazyLib:
+    """The loaded library, opened on first use (so that the build module of
+    this package can be imported before the library exists).  Any use while
+    the library is missing raises ImportError."""
 
-# C-ABI symbols declared in include/fg2_amd.h (checked by the CPU tests)
-EXPORTS = [n for n in dir(lib) if n.startswith("fg2_")]
+    _L = None
+
+    def __getattr__(self, name):
+        if _LazyLib._L is None:
+            _LazyLib._L = _load()
+        return getattr(_LazyLib._L, name)
+
+
+lib = _LazyLib()
 
 
 def _check(code):
@@ -267,6 +278,9 @@ class Run:
 
     def step_async(self, t, dt):
         _check(lib.fg2_step_async(self.h, t, dt))
+
+    def substage(self, stage, dt):
+        _check(lib.fg2_substage(self.h, stage, dt))
 
     def check(self):
         code = lib.fg2_check(self.h)

@@ -2,7 +2,9 @@
 
     python -m fg2_amd.build          # or fg2_amd.build.build()
 
-Flags: -O3 for gfx950 only; packed-FP32 instruction selection is switched
+Flags: -O3 for gfx950 only; -ffp-contract=on, so multiply-adds fuse only
+within one source expression and every kernel that evaluates the same
+expression rounds it the same way; packed-FP32 instruction selection is switched
 off (-Xclang -target-feature -Xclang -packed-fp32-ops), so the compiler
 emits scalar v_fma_f32 / v_add_f32 for the FP32 stencil arithmetic.
 """
@@ -14,13 +16,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfg2_amd.so")
+COPY_EXE = os.path.join(HERE, "hbm_copy")
 SOURCES = ["kernels.hip", "capi.hip"]
 HEADERS = ["fg2_layout.h", "physics.h", "launch.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
     "-I", CSRC, "-I", os.path.join(ROOT, "include"),
@@ -50,6 +53,11 @@ def build(verbose=False, jobs=2):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the HBM copy calibration executable
+    cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "hbm_copy.hip"), "-o", COPY_EXE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     return LIB
 
 

@@ -73,7 +73,7 @@ int module_id(const char *name)
 
 // Largest modified wavenumber of D1 and of -D11, and the RK3 stability
 // limits on the imaginary and negative real axes (|1+z+z^2/2+z^3/6| = 1).
-constexpr double KMAX1 = 1.5866979575;
+constexpr double KMAX1 = 1.5859783962656393;
 constexpr double KMAX2 = 1088.0 / 180.0;
 constexpr double RK3_IMAG = 1.7320508075688772;
 constexpr double RK3_REAL = 2.5127453266183286;
@@ -520,6 +520,18 @@ int fg2_step_async(fg2_run *r, double t, double dt)
   if(int e = check_run(r)) return e;
   randomize(r);
   return substages(r, t, dt, r->dt_mode == FG2_DT_CFL);
+}
+
+int fg2_substage(fg2_run *r, int stage, double dt)
+{
+  if(int e = check_run(r)) return e;
+  if(stage < 0 || stage > 2) return fail(FG2_ERR_ARG, "stage must be 0, 1 or 2");
+  HIP_TRY(launch_substage(r->module, stage, r->in[r->cur], r->in[r->cur ^ 1], r->in[2], r->s,
+                          make_params(r), (float)RK_BETA[stage], (float)(dt * RK_GAMMA[stage]),
+                          shape_of(r), 0, r->diag, r->stream));
+  r->cur ^= 1;
+  r->lam_valid = false;
+  return FG2_OK;
 }
 
 int fg2_check(fg2_run *r)

@@ -128,7 +128,7 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
     addr[0][k - 1] = ((lane + k) & 63) << 2;
     addr[1][k - 1] = ((lane - k) & 63) << 2;
   }
-  const float k1a = 1.5866979575f / P.Delta1, k1b = 1.5866979575f / P.Delta2;
+  const float k1a = 1.5859783962656393f / P.Delta1, k1b = 1.5859783962656393f / P.Delta2;
 
   // register ring: row r lives in slot (r - (i0 - 3)) % 9; rows i-3..i+3
   // are in use, row i+4 is in flight, row i+5 is being requested
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256)
 void cfl_kernel(const float *u, const long s, const Params P, DeviceDiag *diag)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const float k1a = 1.5866979575f / P.Delta1, k1b = 1.5866979575f / P.Delta2;
+  const float k1a = 1.5859783962656393f / P.Delta1, k1b = 1.5859783962656393f / P.Delta2;
   float la = 0.0f, ld = 0.0f;
   if(j < P.n2)
     for(int i = blockIdx.y; i < P.n1; i += gridDim.y) {

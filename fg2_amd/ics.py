@@ -66,13 +66,15 @@ def isot_density(n1, n2, seed=3, kmax=8, rms=0.05):
 
 
 def disk_kepler(n1, n2, seed=4, l1=None, rin=3.0, gm=1.0, h_over_r=0.05,
-                noise=0.01, floor=1e-4, gamma=GAMMA, lner=-30.0):
+                noise=0.01, floor=1e-3, rad_frac=1e-3, gamma=GAMMA):
     """disk: Keplerian disc around a central mass on the (log r, theta) grid
     of src/disk/config.h:21-34.  Sigma ~ R^-1 (rho_mid ~ R^-2) with a
-    Gaussian vertical profile of aspect ratio h_over_r, a density floor,
-    Omega = sqrt(GM / R^3), isothermal sound speed h_over_r * v_K, radial and
-    vertical velocity 0, radiation energy at a floor, and 1% seeded density
-    noise.  Returns (n1, n2, 6)."""
+    Gaussian vertical profile of aspect ratio h_over_r above a density floor
+    of `floor` times the local midplane density, times (1 + noise N(0,1)).
+    Omega = sqrt(GM / r^3) on spheres (the Keplerian value in the midplane;
+    its centrifugal force balances the cylindrical-radial gravity
+    everywhere), isothermal sound speed h_over_r * v_K(R), ur = uz = 0, and
+    radiation energy rad_frac times the thermal energy.  Returns (n1, n2, 6)."""
     rng = np.random.default_rng(seed)
     l2 = np.pi
     if l1 is None:
@@ -84,13 +86,14 @@ def disk_kepler(n1, n2, seed=4, l1=None, rin=3.0, gm=1.0, h_over_r=0.05,
     R = sphr * np.sin(th)[None, :]
     z = sphr * np.cos(th)[None, :]
     H = h_over_r * R
-    rho = R ** -2.0 * np.exp(-0.5 * (z / H) ** 2)
-    rho = np.maximum(rho, floor) * (1.0 + noise * rng.standard_normal((n1, n2)))
-    vk = np.sqrt(gm / R)
-    cs2 = (h_over_r * vk) ** 2
+    rho_mid = R ** -2.0
+    rho = rho_mid * (np.exp(-0.5 * (z / H) ** 2) + floor)
+    rho = rho * (1.0 + noise * rng.standard_normal((n1, n2)))
+    cs2 = h_over_r ** 2 * gm / R
+    e = cs2 / (gamma - 1.0)                      # (gamma - 1) e = c_s^2
     x = np.zeros((n1, n2, 6))
     x[..., 0] = np.log(rho)
-    x[..., 3] = np.sqrt(gm / R ** 3)
-    x[..., 4] = np.log(cs2 / (gamma - 1.0))      # (gamma - 1) e = c_s^2
-    x[..., 5] = lner
+    x[..., 3] = np.sqrt(gm / sphr ** 3) * np.ones_like(R)
+    x[..., 4] = np.log(e)
+    x[..., 5] = np.log(rad_frac * e)
     return x

@@ -138,6 +138,10 @@ int fg2_step(fg2_run *run, double t, double dt);
  * many steps can be queued or captured in a HIP graph; check the flag
  * later with fg2_check. */
 int fg2_step_async(fg2_run *run, double t, double dt);
+/* One fused substage (0, 1 or 2) of an RK3 step of size dt: reads the
+ * state and v, writes v (stages 0, 1) and the new state
+ * (src/step.cpp:30-35 does bcond, kick, drift per substage). */
+int fg2_substage(fg2_run *run, int stage, double dt);
 /* Synchronise the run's stream; FG2_ERR_NAN if any step since the last
  * check produced a NaN (the flag is then cleared). */
 int fg2_check(fg2_run *run);

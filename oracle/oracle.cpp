@@ -537,7 +537,7 @@ template<class R> double getdt(const Sim<R> &S)
 
 // Largest modified wavenumbers of the 6th-order D1 (max over kh of
 // (45 sin kh - 9 sin 2kh + sin 3kh)/30) and of -D11 (at kh = pi: 1088/180).
-constexpr double KMAX1 = 1.5866979575;
+constexpr double KMAX1 = 1.5859783962656393;
 constexpr double KMAX2 = 1088.0 / 180.0;
 // RK3 stability limits on the imaginary and negative real axes:
 // |1 + z + z^2/2 + z^3/6| = 1 at z = i sqrt(3) and at z = -2.5127.
@@ -758,6 +758,26 @@ void orc_rhs(void *p, double *out)
         R f[6];
         rhs_cell(S, &S.u[S.idx(i, j)], i, j, f);
         for(int q = 0; q < S.nv; ++q) out[((size_t)i * S.n2 + j) * S.nv + q] = (double)f[q];
+      }
+    return 0;
+  });
+}
+
+// One operator (0 D1, 1 D2, 2 D11, 3 D22, 4 D12) of every variable over the
+// interior after a boundary fill, into out (.raw order).
+void orc_deriv(void *p, int op, double *out)
+{
+  dispatch((OrcHandle *)p, [&](auto &S) {
+    using R = typename std::remove_reference<decltype(S.u[0])>::type;
+    bcond(S, S.u);
+    for(long i = 0; i < S.n1; ++i)
+      for(long j = 0; j < S.n2; ++j) {
+        Stencil<R> st{&S.u[S.idx(i, j)], S.w * S.nv, S.nv, S.Delta1, S.Delta2};
+        for(int q = 0; q < S.nv; ++q) {
+          R d = op == 0 ? st.d1(q) : op == 1 ? st.d2(q) : op == 2 ? st.d11(q)
+              : op == 3 ? st.d22(q) : st.d12(q);
+          out[((size_t)i * S.n2 + j) * S.nv + q] = (double)d;
+        }
       }
     return 0;
   });

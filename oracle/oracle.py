@@ -87,6 +87,7 @@ def lib():
             "orc_drift": (None, [P, D]),
             "orc_euler": (None, [P, D]),
             "orc_rhs": (None, [P, DP]),
+            "orc_deriv": (None, [P, I, DP]),
             "orc_step": (I, [P, D, D]),
             "orc_run": (I, [P, D, D, I]),
             "orc_getdt": (D, [P]),
@@ -195,6 +196,14 @@ class Oracle:
     def rhs(self):
         out = np.empty(self.shape)
         lib().orc_rhs(self.h, _dp(out))
+        return out
+
+    def deriv(self, op):
+        """op in 'd1', 'd2', 'd11', 'd22', 'd12': the operator of every
+        variable over the interior after a boundary fill."""
+        code = {"d1": 0, "d2": 1, "d11": 2, "d22": 3, "d12": 4}[op]
+        out = np.empty(self.shape)
+        lib().orc_deriv(self.h, code, _dp(out))
         return out
 
     def step(self, t, dt):

@@ -24,6 +24,17 @@ def rel_linf(a, b):
     return num / np.where(den > 1e-30, den, 1.0)
 
 
+# disk parameters of the test states: Newtonian point mass (rS = 0) and a
+# large electron-scattering opacity, so the flux-limited radiative diffusion
+# stays non-stiff in the low-density atmosphere (with the default es the
+# radiation-energy equation needs dt ~ 1e-6 at these resolutions)
+DISK_PARAMS = {"es": 1.0e6, "rS": 0.0}
+
+
+def params_for(module):
+    return dict(DISK_PARAMS) if module == "disk" else None
+
+
 def state(module, n1, n2, seed=1):
     """A smooth seeded state of each module at any size."""
     if module == "hydro":
@@ -32,4 +43,4 @@ def state(module, n1, n2, seed=1):
         x = ics.isot_density(n1, n2, seed=seed)
         x[..., 1:3] = ics.gaussian_random_field(n1, n2, 4, 0.05, seed + 100, ncomp=2)
         return x
-    return ics.disk_kepler(n1, n2, seed=seed)
+    return ics.disk_kepler(n1, n2, seed=seed, h_over_r=0.2)

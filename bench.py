@@ -57,9 +57,10 @@ class HipEvents:
         assert self.hip.hipStreamSynchronize(ctypes.c_void_p(stream)) == 0
 
 
-def load_traffic(nv):
-    """HBM bytes per fused-substage launch from the committed rocprofv3
-    counter summary, scaled by the copy-kernel calibration (None if absent)."""
+def load_traffic():
+    """HBM bytes per launch of the time-step kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload scaled by the
+    copy-kernel calibration (None if the summary is absent)."""
     path = os.path.join(ROOT, "profiles", "fused_substage_hydro_3584_pmc.json")
     if not os.path.exists(path):
         return None
@@ -141,7 +142,7 @@ def main():
     alg_step = 10 * nv * 4 * cells                      # bytes per RK3 step
     launch_s = gpu_ms * 1e-3 / (3 * args.steps)         # average substage launch
     achieved = (alg_step / 3) / launch_s
-    traffic = load_traffic(nv)
+    traffic = load_traffic()
     tile_h, waves = run.tile()
     out = {
         "metric": METRIC,

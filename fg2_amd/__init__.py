@@ -32,7 +32,7 @@ FIELDS = {
 }
 
 OK, ERR_NAN, ERR_HIP, ERR_SHAPE, ERR_ARG, ERR_NOMEM = range(6)
-PATH_FUSED, PATH_PLAIN = 0, 1
+PATH_FUSED, PATH_PLAIN, PATH_SUBSTAGE = 0, 1, 2
 DT_CLOSED, DT_CFL = 0, 1
 
 

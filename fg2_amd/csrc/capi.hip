@@ -71,15 +71,15 @@ int module_id(const char *name)
   return -1;
 }
 
-// Largest modified wavenumber of D1 and of -D11, and the RK3 stability
+// Largest modified wavenumber of -D11 (at kh = pi), and the RK3 stability
 // limits on the imaginary and negative real axes (|1+z+z^2/2+z^3/6| = 1).
-constexpr double KMAX1 = 1.5859783962656393;
+// The D1 counterpart, 1.58598, is applied on the device with the rates.
 constexpr double KMAX2 = 1088.0 / 180.0;
 constexpr double RK3_IMAG = 1.7320508075688772;
 constexpr double RK3_REAL = 2.5127453266183286;
 
-// Williamson low-storage RK3 (src/step.cpp:24-26).
-constexpr double RK_ALPHA[3] = {0.0, 1.0 / 3.0, 3.0 / 4.0};
+// Williamson low-storage RK3 (src/step.cpp:24-26).  The stage times
+// t + alpha dt (alpha = 0, 1/3, 3/4) are not needed: kick ignores t.
 constexpr double RK_BETA[3]  = {0.0, -5.0 / 9.0, -153.0 / 128.0};
 constexpr double RK_GAMMA[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
@@ -124,6 +124,7 @@ struct fg2_run {
   bool lam_valid = false;        // diag holds the CFL rates of the current state
   char para_buf[256];
   size_t bytes = 0;
+  int cus = 256;                 // compute units of the run's device
 };
 
 namespace {
@@ -192,6 +193,30 @@ FusedShape shape_of(const fg2_run *r)
   return sh;
 }
 
+// The one-pass step kernel serves periodic runs of the Cartesian modules.
+bool uses_step_kernel(const fg2_run *r)
+{
+  return r->path == FG2_PATH_FUSED && r->p1 && r->p2 && r->module != DISK;
+}
+
+FusedShape step_shape_of(const fg2_run *r)
+{
+  FusedShape sh = r->shape;
+  if(sh.waves != 1 && sh.waves != 2 && sh.waves != 4) sh.waves = 4;
+  if(r->tile_auto || sh.tile_h < 1) {
+    sh.waves = 4;
+    // one resident round of workgroups: the kernel holds 2 workgroups of 4
+    // wavefronts per CU (its VGPR budget), so aim for 2 x CUs tiles, with at
+    // least 32 rows per tile to bound the 12 recomputed halo rows
+    const long out = 64 * sh.waves - 6 * HALF;
+    const long nstrips = (r->n2 + out - 1) / out;
+    const long chunks = std::max(1L, 2L * r->cus / nstrips);
+    sh.tile_h = (int)std::max(32L, (r->n1 + chunks - 1) / chunks);
+    sh.tile_h = std::min(sh.tile_h, r->n1);
+  }
+  return sh;
+}
+
 // Diffusive rate of the Cartesian modules: D_max (k2/D1^2 + k2/D2^2).
 double lam_diff_const(const fg2_run *r)
 {
@@ -239,8 +264,15 @@ int substages(fg2_run *r, double t, double dt, int cfl)
     r->lam_valid = false;
     return FG2_OK;
   }
-  const FusedShape sh = shape_of(r);
   if(cfl) reset_diag_kernel<<<1, 1, 0, r->stream>>>(r->diag, 1);
+  if(uses_step_kernel(r)) {
+    HIP_TRY(launch_step(r->module, r->in[r->cur], r->in[r->cur ^ 1], r->s, P, dt,
+                        step_shape_of(r), cfl, r->diag, r->stream));
+    r->cur ^= 1;
+    r->lam_valid = cfl != 0;
+    return FG2_OK;
+  }
+  const FusedShape sh = shape_of(r);
   for(int k = 0; k < 3; ++k) {
     HIP_TRY(launch_substage(r->module, k, r->in[r->cur], r->in[r->cur ^ 1], r->in[2], r->s, P,
                             (float)RK_BETA[k], (float)(dt * RK_GAMMA[k]), sh, cfl, r->diag,
@@ -288,6 +320,9 @@ int fg2_setup(fg2_run **out, const char *module, int n1, int n2, size_t *bytes)
   if(e == hipSuccess) e = hipHostMalloc((void **)&r->host_diag, sizeof(DeviceDiag), hipHostMallocDefault);
   if(e == hipSuccess) e = hipMalloc((void **)&r->sum_partial, 4 * (size_t)n1 * sizeof(double));
   if(e == hipSuccess) e = hipMalloc((void **)&r->sum_out, 4 * sizeof(double));
+  int dev = 0;
+  if(e == hipSuccess) e = hipGetDevice(&dev);
+  if(e == hipSuccess) e = hipDeviceGetAttribute(&r->cus, hipDeviceAttributeMultiprocessorCount, dev);
   if(e == hipSuccess) e = hipStreamSynchronize(r->stream);
   if(e != hipSuccess) {
     const int code = (e == hipErrorOutOfMemory) ? FG2_ERR_NOMEM : FG2_ERR_HIP;
@@ -483,7 +518,8 @@ int fg2_getdt_cfl(fg2_run *r, double *dt)
 
 int fg2_set_path(fg2_run *r, int path)
 {
-  if(!r || (path != FG2_PATH_FUSED && path != FG2_PATH_PLAIN)) return fail(FG2_ERR_ARG, "bad path");
+  if(!r || (path != FG2_PATH_FUSED && path != FG2_PATH_PLAIN && path != FG2_PATH_SUBSTAGE))
+    return fail(FG2_ERR_ARG, "bad path");
   r->path = path;
   return FG2_OK;
 }
@@ -509,7 +545,7 @@ int fg2_set_tile(fg2_run *r, int tile_h, int waves)
 int fg2_get_tile(const fg2_run *r, int *tile_h, int *waves)
 {
   if(!r) return fail(FG2_ERR_ARG, "NULL run");
-  const FusedShape sh = shape_of(r);
+  const FusedShape sh = uses_step_kernel(r) ? step_shape_of(r) : shape_of(r);
   if(tile_h) *tile_h = sh.tile_h;
   if(waves) *waves = sh.waves;
   return FG2_OK;

@@ -246,6 +246,145 @@ void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, fl
   commit_tile(r, cfl, diag);
 }
 
+// ---------------------------------------------------------------------------
+// Whole RK3 step in one pass (periodic runs): temporal blocking of the three
+// substages.  A workgroup of NW wavefronts owns 64 NW consecutive columns
+// (3 x 3 halo columns each side are recomputed, so 64 NW - 18 are output)
+// and marches down rows [i0, i1).  At iteration it the workgroup runs
+//   stage 0 on row r0 = i0 - 6 + it   (input u, from HBM, ring W0),
+//   stage 1 on row r0 - 3              (input a = stage-0 output, ring W1),
+//   stage 2 on row r0 - 6              (input b = stage-1 output, ring W2),
+// each stage exactly the per-cell arithmetic of the substage kernel.  The
+// stage v's travel in small register rings (V0, V1); only u is read and the
+// new u written, so HBM traffic per step is 2 NVAR floats per cell.  Rows and
+// columns outside the grid wrap (periodic), and the right-hand side sees the
+// wrapped indices, so every halo value equals the value the substage path
+// computes at the wrapped cell, bit for bit.  Horizontal neighbours of a
+// stage's centre row come from LDS (one s_barrier per stage).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int pmod(int a, int n) { const int r = a % n; return r < 0 ? r + n : r; }
+
+template<class M, int NW>
+__global__ __launch_bounds__(64 * NW)
+void step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const long s,
+                 const Params P, const float g0, const float g1, const float g2,
+                 const float b1, const float b2, const int tile_h, const int nstrips,
+                 const int cfl, DeviceDiag *diag)
+{
+  constexpr int NV = M::NV, ND = M::ND, TW = 64 * NW, H3 = 3 * HALF;
+  __shared__ float X[3][NV][TW];        // centre row of each stage
+  __shared__ float G[3][3][ND][TW];     // g_k = f(i+k) - f(i-k) of each stage
+  const int t = threadIdx.x;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int strip = tile % nstrips, chunk = tile / nstrips;
+  const int n1 = P.n1, n2 = P.n2;
+  const int j0 = strip * (TW - 2 * H3);
+  const int i0 = chunk * tile_h, i1 = min(n1, i0 + tile_h);
+  const int c = j0 - H3 + t;
+  const int cc = pmod(c, n2);                         // this lane's column
+  const bool out = t >= H3 && t < TW - H3 && c < n2;  // lanes of stage-2 output
+  const float *col = uin + (long)cc * NV;
+  const float k1a = 1.5859783962656393f / P.Delta1, k1b = 1.5859783962656393f / P.Delta2;
+  // neighbour indices in LDS rows (clamped: lanes near the workgroup edge
+  // compute garbage that no output depends on)
+  int nb[2][3];
+#pragma unroll
+  for(int k = 1; k <= 3; ++k) { nb[0][k - 1] = min(t + k, TW - 1); nb[1][k - 1] = max(t - k, 0); }
+
+  float W0[8][NV], W1[8][NV], W2[8][NV], V0[4][NV], V1[4][NV];
+  auto load_row = [&](int r, float (&dst)[NV]) {
+    load_cell<NV>(col + (long)pmod(r, n1) * s, dst);
+  };
+#pragma unroll
+  for(int k = 0; k < 7; ++k) load_row(i0 - H3 + k, W0[k]);
+
+  TileOut res{0, 0.0f, 0.0f};
+  const int nit = (i1 - i0) + 4 * HALF;
+
+  // one stage on a centre row whose window rows are win(-3..3)
+  auto stage = [&](const int k, auto &&win, const int i, const float (&vprev)[NV],
+                   const float beta, const float gdt, float (&vn)[NV], float (&uo)[NV]) {
+    const float (&cen)[NV] = win(0);
+#pragma unroll
+    for(int q = 0; q < NV; ++q) X[k][q][t] = cen[q];
+#pragma unroll
+    for(int kk = 1; kk <= 3; ++kk)
+#pragma unroll
+      for(int d = 0; d < ND; ++d)
+        G[k][kk - 1][d][t] = win(kk)[M::D12V[d]] - win(-kk)[M::D12V[d]];
+    __syncthreads();
+    typename M::P_t x;
+#pragma unroll
+    for(int q = 0; q < NV; ++q) x.c[q] = cen[q];
+#pragma unroll
+    for(int kk = 1; kk <= 3; ++kk) {
+#pragma unroll
+      for(int q = 0; q < NV; ++q) {
+        const float a = win(kk)[q], b = win(-kk)[q];
+        x.dv[kk - 1][q] = a - b;
+        x.sv[kk - 1][q] = a + b;
+        const float e = X[k][q][nb[0][kk - 1]], f = X[k][q][nb[1][kk - 1]];
+        x.dh[kk - 1][q] = e - f;
+        x.sh[kk - 1][q] = e + f;
+      }
+#pragma unroll
+      for(int d = 0; d < ND; ++d)
+        x.mx[kk - 1][d] = G[k][kk - 1][d][nb[0][kk - 1]] - G[k][kk - 1][d][nb[1][kk - 1]];
+    }
+    float f[NV];
+    M::rhs(P, x, pmod(i, n1), cc, f);
+#pragma unroll
+    for(int q = 0; q < NV; ++q) {
+      vn[q] = (k == 0) ? f[q] : __builtin_fmaf(beta, vprev[q], f[q]);
+      uo[q] = __builtin_fmaf(gdt, vn[q], cen[q]);
+    }
+  };
+
+  for(int base = 0; base < nit; base += 8) {
+#pragma unroll
+    for(int p = 0; p < 8; ++p) {
+      const int it = base + p;
+      if(it >= nit) break;
+      const int r0 = i0 - 2 * HALF + it;
+      load_row(r0 + 4, W0[(p + 7) & 7]);
+      {                                          // stage 0 on row r0
+        float vn[NV], uo[NV];
+        stage(0, [&](int d) -> float (&)[NV] { return W0[(p + 3 + d) & 7]; }, r0, V0[0],
+              0.0f, g0, vn, uo);
+#pragma unroll
+        for(int q = 0; q < NV; ++q) { W1[p & 7][q] = uo[q]; V0[p & 3][q] = vn[q]; }
+      }
+      if(it >= 2 * HALF) {                       // stage 1 on row r0 - 3
+        float vn[NV], uo[NV];
+        stage(1, [&](int d) -> float (&)[NV] { return W1[(p + 5 + d) & 7]; }, r0 - HALF,
+              V0[(p + 1) & 3], b1, g1, vn, uo);
+#pragma unroll
+        for(int q = 0; q < NV; ++q) { W2[(p + 2) & 7][q] = uo[q]; V1[(p + 2) & 3][q] = vn[q]; }
+      } else {
+        __syncthreads();
+      }
+      if(it >= 4 * HALF) {                       // stage 2 on row r0 - 6
+        float vn[NV], uo[NV];
+        const int r2 = r0 - 2 * HALF;
+        stage(2, [&](int d) -> float (&)[NV] { return W2[(p + 7 + d) & 7]; }, r2,
+              V1[(p + 3) & 3], b2, g2, vn, uo);
+        if(out) {
+#pragma unroll
+          for(int q = 0; q < NV; ++q) res.bad |= (uo[q] != uo[q]);
+          store_cell<NV>(uout + (long)r2 * s + (long)c * NV, uo);
+          if(cfl) {
+            res.lam_adv = fmaxf(res.lam_adv, M::lam_adv(P, uo, r2, c, k1a, k1b));
+            if constexpr(M::CELL_DIFF) res.lam_diff = fmaxf(res.lam_diff, M::lam_diff(P, uo, r2, c));
+          }
+        }
+      } else {
+        __syncthreads();
+      }
+    }
+  }
+  commit_tile(res, cfl, diag);
+}
+
 // Batched substage: a persistent grid of wavefronts pulls tiles of many runs
 // from one atomic counter.  Every tile runs the same process_tile as the
 // single-run kernel, so a run gives the same bits alone and in a batch.
@@ -573,6 +712,37 @@ hipError_t launch_cfl(int module, const float *u, long s, const Params &P, Devic
     const int gy = std::max(1, std::min(P.n1, 2048 / gx));
     cfl_kernel<M><<<dim3(gx, gy), 256, 0, st>>>(u, s, P, diag);
     return hipGetLastError();
+  });
+}
+
+hipError_t launch_step(int module, const float *uin, float *uout, long s, const Params &P,
+                       double dt, const FusedShape &sh, int cfl, DeviceDiag *diag, hipStream_t st)
+{
+  if(!P.p1 || !P.p2) return hipErrorInvalidValue;
+  const float g0 = (float)(dt * (1.0 / 3.0)), g1 = (float)(dt * (15.0 / 16.0)),
+              g2 = (float)(dt * (8.0 / 15.0));
+  const float b1 = (float)(-5.0 / 9.0), b2 = (float)(-153.0 / 128.0);
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    if constexpr(M::ID == DISK) {
+      return hipErrorInvalidValue;   // disk runs are non-periodic: substage path
+    } else {
+    const int tile_h = std::max(1, sh.tile_h);
+    const int nchunks = (P.n1 + tile_h - 1) / tile_h;
+    switch(sh.waves) {
+#define FG2_S(NW)                                                                            \
+    case NW: {                                                                               \
+      const int nstrips = (P.n2 + 64 * NW - 6 * HALF - 1) / (64 * NW - 6 * HALF);            \
+      step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(uin, uout, s, P, g0, g1, g2, \
+                                                              b1, b2, tile_h, nstrips, cfl, diag); \
+      break;                                                                                 \
+    }
+    FG2_S(1) FG2_S(2) FG2_S(4)
+#undef FG2_S
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+    }
   });
 }
 

@@ -29,6 +29,14 @@ hipError_t launch_substage(int module, int stage, const float *uin, float *uout,
                            const FusedShape &shape, int cfl, DeviceDiag *diag,
                            hipStream_t st);
 
+// Whole RK3 step in one launch (both dimensions periodic): the three
+// substages temporally blocked; reads uin, writes uout, v unused.  waves =
+// wavefronts per workgroup (1, 2 or 4); the workgroup recomputes 9 halo
+// columns on each side.
+hipError_t launch_step(int module, const float *uin, float *uout, long s, const Params &P,
+                       double dt, const FusedShape &shape, int cfl, DeviceDiag *diag,
+                       hipStream_t st);
+
 // CFL rates (FP64 max over cells of the advective and diffusive rates)
 // of a state, folded into diag->lam_*_bits.
 hipError_t launch_cfl(int module, const float *u, long s, const Params &P, DeviceDiag *diag,

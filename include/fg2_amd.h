@@ -38,8 +38,12 @@ enum {
 
 /* Step implementation used by fg2_step / fg2_solve. */
 enum {
-  FG2_PATH_FUSED = 0, /* one fused kick+drift launch per RK substage (default)        */
-  FG2_PATH_PLAIN = 1  /* bcond, kick, drift launches per substage (src/step.cpp:30-35) */
+  FG2_PATH_FUSED    = 0, /* default: one launch per RK3 step (all three substages
+                            temporally blocked) when both dimensions are periodic
+                            and the module is hydro or hydro-isot; otherwise one
+                            fused kick+drift launch per substage                     */
+  FG2_PATH_PLAIN    = 1, /* bcond, kick, drift launches per substage (src/step.cpp:30-35) */
+  FG2_PATH_SUBSTAGE = 2  /* one fused kick+drift launch per substage, always         */
 };
 
 /* Time-step rule used by fg2_solve. */
@@ -124,8 +128,10 @@ int fg2_getdt_cfl(fg2_run *run, double *dt);
 /* Path and time-step rule (FG2_PATH_*, FG2_DT_*). */
 int fg2_set_path(fg2_run *run, int path);
 int fg2_set_dt_mode(fg2_run *run, int mode);
-/* Fused-kernel tile: rows per wavefront and wavefronts per workgroup
- * (1, 2 or 4).  tile_h = 0 restores the automatic choice. */
+/* Fused-kernel tile: rows per tile and wavefronts per workgroup (1, 2 or
+ * 4; the one-pass step kernel gives each workgroup 64 waves - 18 output
+ * columns, the substage kernel each wavefront 58).  tile_h = 0 restores the
+ * automatic choice (or the autotuned one, see fg2_autotune). */
 int fg2_set_tile(fg2_run *run, int tile_h, int waves);
 int fg2_get_tile(const fg2_run *run, int *tile_h, int *waves);
 

@@ -100,7 +100,10 @@ template<class R> struct Sim {
     return std::nan("");
   }
   // The reference keeps parameters in R-typed constants, so a parameter is
-  // rounded to R before use (src/para.cpp:33-35).
+  // rounded to R before use (src/para.cpp:33-35).  pr[] holds them in the
+  // order of default_params(), refreshed by refresh() before each sweep.
+  R pr[16] = {};
+  void refresh() { for(size_t k = 0; k < par.size() && k < 16; ++k) pr[k] = (R)par[k].value; }
   R P(const char *name) const { return (R)get(name); }
 };
 
@@ -141,8 +144,8 @@ template<class R> struct Stencil {
 template<class R> void rhs_hydro(const Sim<R> &S, const Stencil<R> &st, R *out)
 {
   enum { LND, U1, U2, LNE };
-  const R g = S.P("gamma"), nus = S.P("nus"), nub = S.P("nub");
-  const R kap = S.P("kappa"), dd = S.P("dd");
+  // gamma nus nub kappa dd
+  const R g = S.pr[0], nus = S.pr[1], nub = S.pr[2], kap = S.pr[3], dd = S.pr[4];
   R d1[4], d2[4], dt[4] = {0, 0, 0, 0};
   for(int q = 0; q < 4; ++q) { d1[q] = st.d1(q); d2[q] = st.d2(q); }
   const R u1 = st.f(U1,0,0), u2 = st.f(U2,0,0);
@@ -187,8 +190,9 @@ template<class R> void rhs_isot(const Sim<R> &S, const Stencil<R> &st,
                                 long i, R *out)
 {
   enum { LND, U1, U2 };
-  const R dd = S.P("dd"), nus = S.P("nus"), nub = S.P("nub");
-  const R cs2 = S.P("cs2"), ff = S.P("f"), kk = S.P("k");
+  // dd nus nub cs2 f k
+  const R dd = S.pr[0], nus = S.pr[1], nub = S.pr[2];
+  const R cs2 = S.pr[3], ff = S.pr[4], kk = S.pr[5];
   R d1[3], d2[3], dt[3] = {0, 0, 0};
   for(int q = 0; q < 3; ++q) { d1[q] = st.d1(q); d2[q] = st.d2(q); }
   const R u1 = st.f(U1,0,0), u2 = st.f(U2,0,0);
@@ -228,10 +232,11 @@ template<class R> void rhs_disk(const Sim<R> &S, const Stencil<R> &st,
                                 long i, long j, R *out)
 {
   enum { LND, UR, UZ, OMG, LNE, LNER };
-  const R M = S.P("M"), rS = S.P("rS"), rin = S.P("rin"), g = S.P("gamma");
-  const R nus = S.P("nus"), nub = S.P("nub"), kap = S.P("kappa");
-  const R alpha = S.P("alpha"), ar = S.P("ar"), es = S.P("es"), ff = S.P("ff");
-  const R pdd = S.P("dd"), pnu = S.P("nu"), ped = S.P("ed"), prd = S.P("rd");
+  // M rS rin gamma nus nub kappa alpha ar es ff dd nu ed rd
+  const R M = S.pr[0], rS = S.pr[1], rin = S.pr[2], g = S.pr[3];
+  const R nus = S.pr[4], nub = S.pr[5], kap = S.pr[6];
+  const R alpha = S.pr[7], ar = S.pr[8], es = S.pr[9], ff = S.pr[10];
+  const R pdd = S.pr[11], pnu = S.pr[12], ped = S.pr[13], prd = S.pr[14];
 
   const R theta = ((R)j + (R)0.5) * S.Delta2;                       // :48-54
   const R sint = std::sin(theta), cost = std::cos(theta);
@@ -409,6 +414,7 @@ template<class R> void rhs_cell(const Sim<R> &S, const R *c, long i, long j, R *
 // kick: v <- F(u) + beta v over the interior; src/kick.cu:41-57,98.
 template<class R> void kick(Sim<R> &S, R beta)
 {
+  S.refresh();
   const int nv = S.nv;
   #pragma omp parallel for schedule(static)
   for(long i = 0; i < S.n1; ++i) {
@@ -752,6 +758,7 @@ void orc_rhs(void *p, double *out)
   dispatch((OrcHandle *)p, [&](auto &S) {
     using R = typename std::remove_reference<decltype(S.u[0])>::type;
     bcond(S, S.u);
+    S.refresh();
     #pragma omp parallel for schedule(static)
     for(long i = 0; i < S.n1; ++i)
       for(long j = 0; j < S.n2; ++j) {
@@ -770,6 +777,7 @@ void orc_deriv(void *p, int op, double *out)
   dispatch((OrcHandle *)p, [&](auto &S) {
     using R = typename std::remove_reference<decltype(S.u[0])>::type;
     bcond(S, S.u);
+    S.refresh();
     for(long i = 0; i < S.n1; ++i)
       for(long j = 0; j < S.n2; ++j) {
         Stencil<R> st{&S.u[S.idx(i, j)], S.w * S.nv, S.nv, S.Delta1, S.Delta2};

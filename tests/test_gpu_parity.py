@@ -175,3 +175,21 @@ def test_batch_equals_alone(fg2):
         assert r.run(0.0, d, 4) == 0
     for r, q in zip(runs2, runs3):
         np.testing.assert_array_equal(r.download(), q.download())
+
+
+@pytest.mark.parametrize("module", ["hydro", "hydro-isot"])
+@pytest.mark.parametrize("n1,n2", [(40, 50), (130, 300), (257, 251)])
+def test_step_kernel_equals_substages(fg2, module, n1, n2):
+    """The one-pass RK3 step (temporally blocked, periodic runs) gives the
+    bits of three substage launches, for any tile and workgroup shape,
+    including grids narrower than one workgroup strip (wrapped halos)."""
+    _say(f"case step kernel {module} {n1}x{n2}")
+    x = state(module, n1, n2, seed=21)
+    ref = _run(fg2, module, n1, n2, x, fg2.PATH_SUBSTAGE)
+    dt = 0.25 * ref.getdt()
+    assert ref.run(0.0, dt, 3) == 0
+    y = ref.download()
+    for tile in [(0, 4), (8, 1), (37, 2), (128, 4), (300, 1)]:
+        r = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, tile=tile)
+        assert r.run(0.0, dt, 3) == 0
+        np.testing.assert_array_equal(r.download(), y)

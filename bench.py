@@ -93,6 +93,130 @@ def cpu_baseline(budget_s=15.0):
                       f"({el:.1f} s, OMP_NUM_THREADS={threads})"}
 
 
+def log_uniform_sizes(rng, count, lo, hi, mult):
+    """Grid sides drawn log-uniform on [lo, hi], rounded to a multiple of mult."""
+    v = np.exp(rng.uniform(np.log(lo), np.log(hi), count))
+    return np.clip((np.round(v / mult) * mult).astype(int), lo, hi)
+
+
+def bench_batched(ev, steps=500, nruns=64, seed=7):
+    """64 independent hydro-isot runs (c_s = 1, box 2 pi) with sides
+    log-uniform in [128, 1024], multiples of 32, seeded Gaussian density
+    perturbations (delta rho / rho = 0.05, |k| <= 8), `steps` RK3 steps each.
+    Batched: one persistent launch per step for all runs (tile shape from the
+    autotuner), captured in a HIP graph.  Baseline: each run launching its own
+    single-run kernel, issued round-robin on the runs' own streams."""
+    import fg2_amd
+    from fg2_amd import ics
+    rng = np.random.default_rng(seed)
+    n1s = log_uniform_sizes(rng, nruns, 128, 1024, 32)
+    n2s = log_uniform_sizes(rng, nruns, 128, 1024, 32)
+    xs = [ics.isot_density(a, b, seed=seed + 1 + k) for k, (a, b) in enumerate(zip(n1s, n2s))]
+    cells = int(np.sum(n1s.astype(np.int64) * n2s))
+
+    runs = [fg2_amd.Run("hydro-isot", int(a), int(b)) for a, b in zip(n1s, n2s)]
+    for r, x in zip(runs, xs):
+        r.upload(x)
+    dts = [r.getdt() for r in runs]
+    # per-run launches
+    for r, d in zip(runs, dts):
+        r.step_async(0.0, d)
+    for r in runs:
+        ev.sync(r.stream())
+    t0 = time.perf_counter()
+    for k in range(steps):
+        for r, d in zip(runs, dts):
+            r.step_async(k * d, d)
+    for r in runs:
+        ev.sync(r.stream())
+    t_plain = time.perf_counter() - t0
+    for r in runs:
+        assert r.check() == 0
+        r.close()
+
+    runs = [fg2_amd.Run("hydro-isot", int(a), int(b)) for a, b in zip(n1s, n2s)]
+    for r, x in zip(runs, xs):
+        r.upload(x)
+    batch = fg2_amd.Batch(runs, waves=4, tile_h=0)
+    tile = batch.autotune(dts)
+    batch.graph_run(dts, steps=2, replays=1)          # warm-up (and graph build)
+    t0 = time.perf_counter()
+    batch.graph_run(dts, steps=steps, replays=1)
+    t_batch = time.perf_counter() - t0
+    bad = sum(r.check() != 0 for r in runs)
+    batch.close()
+    for r in runs:
+        r.close()
+    return {"workload": f"{nruns} hydro-isot runs, sides log-uniform 128..1024 (x32), "
+                        f"{steps} RK3 steps each, FP32",
+            "cells": cells, "steps": steps,
+            "batched_cell_updates_per_s": cells * steps / t_batch,
+            "per_run_cell_updates_per_s": cells * steps / t_plain,
+            "speedup": t_plain / t_batch, "batched_s": t_batch, "per_run_s": t_plain,
+            "tile": {"tile_h": tile[0], "waves": tile[1]}, "nan_runs": bad,
+            "graph": "one hipGraph of all steps (graph build included in the timing)"}
+
+
+def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=2, tile_h=64):
+    """Poisson stream of run requests (rate per second) mixing hydro and
+    hydro-isot, sides log-uniform in 64..2048, steps uniform in 50..2000,
+    seeded random-field initial states; continuous batching under an HBM
+    budget.  Reports p50 / p99 completion latency and aggregate
+    cell-updates/s."""
+    import fg2_amd
+    from fg2_amd import ics
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    mods = rng.integers(0, 2, nreq)
+    n1s = log_uniform_sizes(rng, nreq, 64, 2048, 1)
+    n2s = log_uniform_sizes(rng, nreq, 64, 2048, 1)
+    steps = rng.integers(50, 2001, nreq)
+    # initial states are generated before the timed loop (host set-up)
+    states = []
+    for k in range(nreq):
+        a, b = int(n1s[k]), int(n2s[k])
+        if mods[k] == 0:
+            states.append(ics.hydro_random(a, b, seed=1000 + k, dtype=np.float32))
+        else:
+            states.append(ics.isot_density(a, b, seed=1000 + k, dtype=np.float32))
+    srv = fg2_amd.Server(int(budget_gib * 2 ** 30), waves=waves, tile_h=tile_h)
+    ids = []
+    k = 0
+    ticks, tick_s = 0, 0.0
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        while k < nreq and arrivals[k] <= now:
+            m = "hydro" if mods[k] == 0 else "hydro-isot"
+            ids.append(srv.submit(m, states[k], int(steps[k]), arrivals[k]))
+            states[k] = None
+            k += 1
+        th0 = time.perf_counter()
+        act, que, fin = srv.tick(th0 - t0)
+        tick_s += time.perf_counter() - th0
+        ticks += 1
+        if k == nreq and fin == nreq:
+            break
+        if act == 0 and que == 0 and k < nreq:
+            time.sleep(max(0.0, arrivals[k] - (time.perf_counter() - t0)))
+    wall = time.perf_counter() - t0
+    lat, bad = [], 0
+    for i in ids:
+        st, (ta, tadm, td), _ = srv.result(i, fetch=False)
+        lat.append(td - ta)
+        bad += st != 0
+    cell_steps, _ = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    return {"workload": f"{nreq} requests, Poisson {rate:g}/s, hydro + hydro-isot, "
+                        "sides log-uniform 64..2048, steps U[50,2000], FP32",
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "cell_updates_per_s": cell_steps / wall,
+            "wall_s": wall, "ticks": ticks, "host_s_in_tick": tick_s,
+            "hbm_budget_gib": budget_gib, "waves": waves, "tile_h": tile_h,
+            "nan_requests": int(bad)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,7 +224,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--waves", type=int, default=1)
+    ap.add_argument("--waves", type=int, default=4)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -169,9 +295,15 @@ def main():
                      "kernel": "fused substage (mean of the 3 stage launches)",
                      "alg_bytes_per_launch": alg_step / 3,
                      "avg_launch_us": launch_s * 1e6},
+        "batched": None,
+        "serving": None,
         "cpu_baseline": None,
     }
     run.close()
+    if not args.no_batch:
+        out["batched"] = bench_batched(ev)
+    if not args.no_serve:
+        out["serving"] = bench_serving()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

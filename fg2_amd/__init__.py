@@ -83,6 +83,7 @@ def _load():
         "fg2_set_dt_mode": (I, [P, I]),
         "fg2_set_tile": (I, [P, I, I]),
         "fg2_get_tile": (I, [P, IP, IP]),
+        "fg2_autotune": (I, [P, IP, IP]),
         "fg2_step": (I, [P, D, D]),
         "fg2_step_async": (I, [P, D, D]),
         "fg2_substage": (I, [P, I, D]),
@@ -96,6 +97,14 @@ def _load():
         "fg2_batch_step": (I, [P, DP]),
         "fg2_batch_graph_run": (I, [P, DP, I, I]),
         "fg2_batch_stream": (P, [P]),
+        "fg2_batch_autotune": (I, [P, DP, IP, IP]),
+        "fg2_server_create": (I, [PP, ctypes.c_size_t, I, I]),
+        "fg2_server_free": (None, [P]),
+        "fg2_server_submit": (I, [P, ctypes.c_char_p, I, I, I, FP, D,
+                                  ctypes.POINTER(ctypes.c_longlong)]),
+        "fg2_server_tick": (I, [P, D, IP, IP, IP]),
+        "fg2_server_result": (I, [P, ctypes.c_longlong, FP, DP, IP]),
+        "fg2_server_stats": (I, [P, ctypes.POINTER(ctypes.c_longlong), SP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -219,6 +228,11 @@ class Run:
 
     def set_tile(self, tile_h, waves=1):
         _check(lib.fg2_set_tile(self.h, int(tile_h), int(waves)))
+
+    def autotune(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib.fg2_autotune(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def tile(self):
         a, b = ctypes.c_int(), ctypes.c_int()
@@ -351,3 +365,56 @@ class Batch:
 
     def stream(self):
         return lib.fg2_batch_stream(self.h)
+
+    def autotune(self, dts):
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib.fg2_batch_autotune(self.h, _dp(d), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+
+class Server:
+    """Continuous-batching request loop over the C ABI (fg2_server_*)."""
+
+    def __init__(self, hbm_budget, waves=4, tile_h=64):
+        h = ctypes.c_void_p()
+        _check(lib.fg2_server_create(ctypes.byref(h), int(hbm_budget), waves, tile_h))
+        self.h = h
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.fg2_server_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, module, init, steps, now):
+        x = np.ascontiguousarray(init, dtype=np.float32)
+        n1, n2 = x.shape[0], x.shape[1]
+        i = ctypes.c_longlong()
+        _check(lib.fg2_server_submit(self.h, module.encode(), n1, n2, int(steps), _fp(x),
+                                     float(now), ctypes.byref(i)))
+        self.shapes[i.value] = x.shape
+        return i.value
+
+    def tick(self, now):
+        a, q, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib.fg2_server_tick(self.h, float(now), ctypes.byref(a), ctypes.byref(q),
+                                   ctypes.byref(f)))
+        return a.value, q.value, f.value
+
+    def result(self, rid, fetch=True):
+        """(status, (t_arrive, t_admit, t_done), state or None)."""
+        t = np.zeros(3)
+        st = ctypes.c_int()
+        out = np.empty(self.shapes[rid], np.float32) if fetch else None
+        _check(lib.fg2_server_result(self.h, rid, _fp(out) if fetch else None, _dp(t),
+                                     ctypes.byref(st)))
+        return st.value, tuple(t.tolist()), (out if st.value >= 0 else None)
+
+    def stats(self):
+        c, b = ctypes.c_longlong(), ctypes.c_size_t()
+        _check(lib.fg2_server_stats(self.h, ctypes.byref(c), ctypes.byref(b)))
+        return c.value, b.value

@@ -16,6 +16,9 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "fg2_amd.h"
 #include "launch.h"
@@ -551,6 +554,93 @@ int fg2_get_tile(const fg2_run *r, int *tile_h, int *waves)
   return FG2_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner (replaces the device-property heuristic of src/setup.cpp:39-58):
+// times the run's own kernel on its own state for candidate shapes and keeps
+// the fastest.  The timed launches read the state and write only the other
+// state buffer and v, so the state is unchanged.  Results are cached per
+// (module, n1, n2, kernel) for the process.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TuneKey {
+  int module, n1, n2, step;
+  bool operator<(const TuneKey &o) const
+  { return std::tie(module, n1, n2, step) < std::tie(o.module, o.n1, o.n2, o.step); }
+};
+std::map<TuneKey, FusedShape> g_tune_cache;
+std::mutex g_tune_mutex;
+
+int time_launch(fg2_run *r, const FusedShape &sh, bool step, float *ms)
+{
+  const Params P = make_params(r);
+  const double dt = 1e-7;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  auto once = [&]() -> hipError_t {
+    if(step) return launch_step(r->module, r->in[r->cur], r->in[r->cur ^ 1], r->s, P, dt, sh, 0,
+                                r->diag, r->stream);
+    return launch_substage(r->module, 1, r->in[r->cur], r->in[r->cur ^ 1], r->in[2], r->s, P,
+                           (float)RK_BETA[1], (float)(dt * RK_GAMMA[1]), sh, 0, r->diag, r->stream);
+  };
+  HIP_TRY(once());
+  HIP_TRY(hipEventRecord(e0, r->stream));
+  for(int k = 0; k < 3; ++k) HIP_TRY(once());
+  HIP_TRY(hipEventRecord(e1, r->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return FG2_OK;
+}
+
+} // namespace
+
+extern "C" int fg2_autotune(fg2_run *r, int *tile_h, int *waves)
+{
+  if(int e = check_run(r)) return e;
+  if(r->path == FG2_PATH_PLAIN) return fail(FG2_ERR_ARG, "the plain path has no tile shape");
+  const bool step = uses_step_kernel(r);
+  const TuneKey key{r->module, r->n1, r->n2, step ? 1 : 0};
+  FusedShape best{0, 0};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    auto it = g_tune_cache.find(key);
+    if(it != g_tune_cache.end()) best = it->second;
+  }
+  if(best.tile_h == 0) {
+    std::vector<FusedShape> cand;
+    const FusedShape a = step ? step_shape_of(r) : shape_of(r);
+    cand.push_back(a);
+    const int hs_step[] = {32, 48, 64, 96, 112, 128, 160, 224};
+    const int hs_sub[] = {16, 24, 32, 48, 64, 128};
+    if(step) {
+      for(int w : {4, 2})
+        for(int h : hs_step) if(h <= r->n1) cand.push_back({h, w});
+    } else {
+      for(int w : {4, 2, 1})
+        for(int h : hs_sub) if(h <= r->n1) cand.push_back({h, w});
+    }
+    float best_ms = 1e30f;
+    for(const FusedShape &c : cand) {
+      float ms;
+      if(int e = time_launch(r, c, step, &ms)) return e;
+      if(ms < best_ms) { best_ms = ms; best = c; }
+    }
+    reset_diag_kernel<<<1, 1, 0, r->stream>>>(r->diag, 0);
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    g_tune_cache[key] = best;
+  }
+  r->shape = best;
+  r->tile_auto = false;
+  r->lam_valid = false;
+  if(tile_h) *tile_h = best.tile_h;
+  if(waves) *waves = best.waves;
+  return FG2_OK;
+}
+
 int fg2_step_async(fg2_run *r, double t, double dt)
 {
   if(int e = check_run(r)) return e;
@@ -642,6 +732,8 @@ int fg2_perf_model(const fg2_run *r, double *flops, double *bits, double *alg)
 // ---------------------------------------------------------------------------
 struct fg2_batch {
   int module = 0, nruns = 0, waves = 1, grid = 0, total = 0;
+  bool step = false;          // one-pass step kernel (all runs periodic) or substages
+  int cus = 256;
   std::vector<fg2_run *> runs;
   std::vector<int> tile_h;
   BatchRun *dev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};  // [stage][parity]
@@ -658,8 +750,41 @@ __global__ void zero_ints_kernel(int *p, int n)
   if((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+int batch_fill_step(fg2_batch *b, const double *dts)
+{
+  const int out = 64 * b->waves - 6 * HALF;
+  for(int par = 0; par < 2; ++par) {
+    auto &H = b->host[0][par];
+    H.resize(b->nruns);
+    int begin = 0;
+    for(int i = 0; i < b->nruns; ++i) {
+      fg2_run *r = b->runs[i];
+      const int cur = r->cur ^ par;
+      BatchRun &R = H[i];
+      R = BatchRun{};
+      R.uin = r->in[cur];
+      R.uout = r->in[cur ^ 1];
+      R.v = r->in[2];
+      R.s = r->s;
+      R.P = make_params(r);
+      R.tile_h = b->tile_h[i];
+      R.nstrips = (r->n2 + out - 1) / out;
+      R.ntiles = R.nstrips * ((r->n1 + R.tile_h - 1) / R.tile_h);
+      R.tile_begin = begin;
+      begin += R.ntiles;
+      for(int k = 0; k < 3; ++k) R.g[k] = (float)(dts[i] * RK_GAMMA[k]);
+      R.diag = r->diag;
+    }
+    b->total = begin;
+    HIP_TRY(hipMemcpyAsync(b->dev[0][par], H.data(), b->nruns * sizeof(BatchRun),
+                           hipMemcpyHostToDevice, b->stream));
+  }
+  return FG2_OK;
+}
+
 int batch_fill(fg2_batch *b, const double *dts)
 {
+  if(b->step) return batch_fill_step(b, dts);
   // stage k reads u[cur ^ (k & 1)] and writes u[cur ^ 1 ^ (k & 1)] of each
   // run, where cur is the run's state buffer at the start of the step
   for(int par = 0; par < 2; ++par)
@@ -695,6 +820,11 @@ int batch_fill(fg2_batch *b, const double *dts)
 int batch_launch(fg2_batch *b, int par)
 {
   zero_ints_kernel<<<1, 64, 0, b->stream>>>(b->counters, 3);
+  if(b->step) {
+    HIP_TRY(launch_batch_step(b->module, b->dev[0][par], b->nruns, b->total, b->waves,
+                              b->grid, b->counters, b->stream));
+    return FG2_OK;
+  }
   for(int k = 0; k < 3; ++k)
     HIP_TRY(launch_batch_substage(b->module, k, b->dev[k][par], b->nruns, b->total, b->waves,
                                   b->grid, b->counters + k, 0, b->stream));
@@ -715,11 +845,16 @@ int fg2_batch_create(fg2_batch **out, fg2_run *const *runs, int nruns, int waves
   b->module = runs[0]->module;
   b->nruns = nruns;
   b->waves = waves;
+  b->step = true;
   for(int i = 0; i < nruns; ++i) {
     if(int e = check_run(runs[i])) { delete b; return e; }
     if(runs[i]->module != b->module) { delete b; return fail(FG2_ERR_ARG, "a batch holds runs of one module"); }
+    b->step = b->step && runs[i]->p1 && runs[i]->p2 && runs[i]->module != DISK;
+  }
+  for(int i = 0; i < nruns; ++i) {
     b->runs.push_back(runs[i]);
-    b->tile_h.push_back(tile_h > 0 ? std::min(tile_h, runs[i]->n1) : shape_of(runs[i]).tile_h);
+    const int auto_h = b->step ? 64 : shape_of(runs[i]).tile_h;
+    b->tile_h.push_back(std::min(tile_h > 0 ? tile_h : auto_h, runs[i]->n1));
   }
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   for(int k = 0; k < 3 && e == hipSuccess; ++k)
@@ -730,8 +865,10 @@ int fg2_batch_create(fg2_batch **out, fg2_run *const *runs, int nruns, int waves
   if(e == hipSuccess) e = hipGetDevice(&dev);
   if(e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if(e != hipSuccess) { fg2_batch_free(b); return fail(FG2_ERR_HIP, "batch setup: %s", hipGetErrorString(e)); }
-  // persistent grid: 16 wavefronts per CU
-  b->grid = cus * 16 / waves;
+  // persistent grid: one resident round of workgroups (the step kernel's
+  // VGPR budget allows 8 wavefronts per CU, the substage kernel's 16)
+  b->cus = cus;
+  b->grid = b->step ? cus * 8 / waves : cus * 16 / waves;
   *out = b;
   return FG2_OK;
 }
@@ -748,6 +885,50 @@ void fg2_batch_free(fg2_batch *b)
 }
 
 void *fg2_batch_stream(fg2_batch *b) { return b ? (void *)b->stream : nullptr; }
+
+// Time one batched step (the records write every run's other state buffer,
+// which the next real step overwrites) for candidate (tile_h, waves) and
+// keep the fastest; the same tile height is used for every run.
+int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
+{
+  if(!b || !dts) return fail(FG2_ERR_ARG, "NULL argument");
+  for(fg2_run *r : b->runs) HIP_TRY(hipStreamSynchronize(r->stream));
+  std::vector<FusedShape> cand;
+  const int hs[] = {16, 32, 48, 64, 96, 128};
+  const int ws_step[] = {4, 2}, ws_sub[] = {4, 2, 1};
+  if(b->step) { for(int w : ws_step) for(int h : hs) cand.push_back({h, w}); }
+  else        { for(int w : ws_sub)  for(int h : hs) cand.push_back({h, w}); }
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  float best_ms = 1e30f;
+  FusedShape best{0, 0};
+  for(const FusedShape &c : cand) {
+    b->waves = c.waves;
+    b->grid = b->step ? b->cus * 8 / c.waves : b->cus * 16 / c.waves;
+    for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = std::min(c.tile_h, b->runs[i]->n1);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if(int e = batch_fill(b, dts)) return e;
+    if(int e = batch_launch(b, 0)) return e;
+    HIP_TRY(hipEventRecord(e0, b->stream));
+    for(int k = 0; k < 2; ++k) if(int e = batch_launch(b, 0)) return e;
+    HIP_TRY(hipEventRecord(e1, b->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if(ms < best_ms) { best_ms = ms; best = c; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->waves = best.waves;
+  b->grid = b->step ? b->cus * 8 / best.waves : b->cus * 16 / best.waves;
+  for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = std::min(best.tile_h, b->runs[i]->n1);
+  for(fg2_run *r : b->runs) reset_diag_kernel<<<1, 1, 0, b->stream>>>(r->diag, 0);
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if(tile_h) *tile_h = best.tile_h;
+  if(waves) *waves = best.waves;
+  return FG2_OK;
+}
 
 int fg2_batch_step_async(fg2_batch *b, const double *dts)
 {
@@ -818,6 +999,395 @@ int fg2_batch_graph_run(fg2_batch *b, const double *dts, int steps, int replays)
   }
   HIP_TRY(hipGraphExecDestroy(ge));
   HIP_TRY(hipGraphDestroy(g));
+  return FG2_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching of many runs of different sizes.
+//
+// Requests (module, grid, step count, initial state) queue on submit.  Each
+// tick retires runs whose copy-back has completed, admits queued requests in
+// arrival order while their state fits the HBM budget, and advances every
+// active run by one RK3 step with one batched one-pass launch per module.
+// State stays on the device from admission to retirement, in an arena
+// allocated once (first-fit with coalescing), so serving makes no hipMalloc
+// or hipFree calls; a served run holds only its two state arrays (the one-
+// pass step needs no v).  Launch records are double-buffered in pinned
+// memory behind events, so the host prepares tick k+1 while tick k runs.
+// A run's arithmetic is that of fg2_step (the same step tile), so a result
+// is the same bits alone and in any mix.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Request {
+  long long id;
+  int module, n1, n2, nv, steps, done_steps = 0;
+  long s = 0;
+  std::vector<float> host;             // initial state, then the result
+  double t_arrive = 0, t_admit = -1, t_done = -1;
+  size_t off = 0, bytes = 0;           // arena region: two state arrays
+  float *in[2] = {nullptr, nullptr};
+  int cur = 0, diag_slot = -1;
+  Params P{};
+  double dt = 0;
+  int status = FG2_OK;
+  bool active = false, retiring = false;
+  hipEvent_t copied = nullptr;
+};
+
+} // namespace
+
+struct fg2_server {
+  size_t budget = 0, in_use = 0;
+  int waves = 4, tile_h = 64, cus = 256;
+  hipStream_t stream = nullptr;
+  char *arena = nullptr;
+  std::map<size_t, size_t> free_list;          // offset -> size
+  DeviceDiag *diags = nullptr;
+  DeviceDiag *diags_host = nullptr;            // pinned readback of retired slots
+  int ndiag = 0;
+  std::vector<int> free_diag;
+  long long next_id = 0;
+  std::vector<Request *> queue, active, finished;
+  std::map<long long, Request *> all;
+  static constexpr int SLOTS = 4;              // launch records in flight
+  BatchRun *rec_dev[SLOTS] = {};
+  BatchRun *rec_pinned[SLOTS] = {};
+  size_t rec_cap = 0;
+  hipEvent_t rec_done[SLOTS] = {};
+  int rec_slot = 0;
+  int *counters = nullptr;                     // one per slot
+  std::vector<BatchRun> rec_host;
+  long long cell_steps = 0;
+  // host <-> device copies of admitted / retired states run on their own
+  // stream through pinned staging buffers, so they overlap the steps
+  hipStream_t copy = nullptr;
+  static constexpr int STAGES = 4;
+  float *up[STAGES] = {}, *down[STAGES] = {};
+  size_t up_cap[STAGES] = {}, down_cap[STAGES] = {};
+  hipEvent_t up_done[STAGES] = {}, down_done[STAGES] = {};
+  Request *down_owner[STAGES] = {};
+  int up_slot = 0;
+};
+
+namespace {
+
+int stage_buffer(float **buf, size_t *cap, size_t need)
+{
+  if(need <= *cap) return FG2_OK;
+  if(*buf) HIP_TRY(hipHostFree(*buf));
+  *buf = nullptr;
+  const size_t c = std::max(need, (size_t)1 << 24);
+  HIP_TRY(hipHostMalloc((void **)buf, c, hipHostMallocDefault));
+  *cap = c;
+  return FG2_OK;
+}
+
+bool arena_alloc(fg2_server *S, size_t bytes, size_t *off)
+{
+  bytes = (bytes + 255) / 256 * 256;
+  for(auto it = S->free_list.begin(); it != S->free_list.end(); ++it)
+    if(it->second >= bytes) {
+      *off = it->first;
+      const size_t rest = it->second - bytes, at = it->first + bytes;
+      S->free_list.erase(it);
+      if(rest) S->free_list[at] = rest;
+      return true;
+    }
+  return false;
+}
+
+void arena_free(fg2_server *S, size_t off, size_t bytes)
+{
+  bytes = (bytes + 255) / 256 * 256;
+  auto it = S->free_list.emplace(off, bytes).first;
+  auto nx = std::next(it);
+  if(nx != S->free_list.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    S->free_list.erase(nx);
+  }
+  if(it != S->free_list.begin()) {
+    auto pv = std::prev(it);
+    if(pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      S->free_list.erase(it);
+    }
+  }
+}
+
+int server_grow(fg2_server *S, size_t need)
+{
+  if(need <= S->rec_cap) return FG2_OK;
+  size_t cap = std::max<size_t>(need, 2 * S->rec_cap + 64);
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  for(int k = 0; k < fg2_server::SLOTS; ++k) {
+    if(S->rec_dev[k]) HIP_TRY(hipFree(S->rec_dev[k]));
+    if(S->rec_pinned[k]) HIP_TRY(hipHostFree(S->rec_pinned[k]));
+    HIP_TRY(hipMalloc((void **)&S->rec_dev[k], cap * sizeof(BatchRun)));
+    HIP_TRY(hipHostMalloc((void **)&S->rec_pinned[k], cap * sizeof(BatchRun), hipHostMallocDefault));
+  }
+  S->rec_cap = cap;
+  return FG2_OK;
+}
+
+int server_admit(fg2_server *S, double now)
+{
+  while(!S->queue.empty() && !S->free_diag.empty()) {
+    Request *q = S->queue.front();
+    const size_t one = (size_t)array_floats(q->n1, q->n2, q->nv) * sizeof(float);
+    size_t off;
+    if(!arena_alloc(S, 2 * one, &off)) break;          // wait for retirements
+    q->off = off;
+    q->bytes = 2 * one;
+    q->s = row_stride(q->n2, q->nv);
+    const long io = interior_offset(q->n2, q->nv);
+    q->in[0] = (float *)(S->arena + off) + io;
+    q->in[1] = (float *)(S->arena + off + one) + io;
+    q->diag_slot = S->free_diag.back();
+    S->free_diag.pop_back();
+    HIP_TRY(hipMemsetAsync(S->diags + q->diag_slot, 0, sizeof(DeviceDiag), S->copy));
+    const size_t hp = (size_t)q->n2 * q->nv * sizeof(float);
+    const int u = S->up_slot;
+    S->up_slot = (S->up_slot + 1) % fg2_server::STAGES;
+    HIP_TRY(hipEventSynchronize(S->up_done[u]));                // staging slot free
+    if(int e = stage_buffer(&S->up[u], &S->up_cap[u], hp * q->n1)) return e;
+    memcpy(S->up[u], q->host.data(), hp * q->n1);
+    HIP_TRY(hipMemcpy2DAsync(q->in[0], q->s * sizeof(float), S->up[u], hp, hp, q->n1,
+                             hipMemcpyHostToDevice, S->copy));
+    HIP_TRY(hipEventRecord(S->up_done[u], S->copy));
+    HIP_TRY(hipStreamWaitEvent(S->stream, S->up_done[u], 0));  // steps after the upload
+    q->t_admit = now;
+    q->active = true;
+    S->in_use += q->bytes;
+    S->queue.erase(S->queue.begin());
+    S->active.push_back(q);
+  }
+  return FG2_OK;
+}
+
+int server_launch(fg2_server *S)
+{
+  const int out = 64 * S->waves - 6 * HALF;
+  for(int m = 0; m < 2; ++m) {
+    S->rec_host.clear();
+    int begin = 0;
+    for(Request *q : S->active) {
+      if(q->module != m || q->done_steps >= q->steps) continue;
+      BatchRun R{};
+      R.uin = q->in[q->cur];
+      R.uout = q->in[q->cur ^ 1];
+      R.s = q->s;
+      R.P = q->P;
+      R.tile_h = std::min(S->tile_h, q->n1);
+      R.nstrips = (q->n2 + out - 1) / out;
+      R.ntiles = R.nstrips * ((q->n1 + R.tile_h - 1) / R.tile_h);
+      R.tile_begin = begin;
+      begin += R.ntiles;
+      for(int k = 0; k < 3; ++k) R.g[k] = (float)(q->dt * RK_GAMMA[k]);
+      R.diag = S->diags + q->diag_slot;
+      S->rec_host.push_back(R);
+    }
+    if(S->rec_host.empty()) continue;
+    if(int e = server_grow(S, S->rec_host.size())) return e;
+    const int slot = S->rec_slot;
+    S->rec_slot = (S->rec_slot + 1) % fg2_server::SLOTS;
+    HIP_TRY(hipEventSynchronize(S->rec_done[slot]));   // the slot's previous launch is done
+    memcpy(S->rec_pinned[slot], S->rec_host.data(), S->rec_host.size() * sizeof(BatchRun));
+    HIP_TRY(hipMemcpyAsync(S->rec_dev[slot], S->rec_pinned[slot], S->rec_host.size() * sizeof(BatchRun),
+                           hipMemcpyHostToDevice, S->stream));
+    zero_ints_kernel<<<1, 64, 0, S->stream>>>(S->counters + slot, 1);
+    HIP_TRY(launch_batch_step(m, S->rec_dev[slot], (int)S->rec_host.size(), begin, S->waves,
+                              S->cus * 8 / S->waves, S->counters + slot, S->stream));
+    HIP_TRY(hipEventRecord(S->rec_done[slot], S->stream));
+  }
+  for(Request *q : S->active) {
+    if(q->done_steps >= q->steps) continue;
+    q->cur ^= 1;
+    ++q->done_steps;
+    S->cell_steps += (long long)q->n1 * q->n2;
+  }
+  return FG2_OK;
+}
+
+int server_retire(fg2_server *S, double now)
+{
+  // queue the copy-back (on the copy stream, after the run's last step) of
+  // runs that took their last step, one pinned staging slot each
+  for(Request *q : S->active)
+    if(!q->retiring && q->done_steps >= q->steps) {
+      int d = -1;
+      for(int k = 0; k < fg2_server::STAGES; ++k) if(!S->down_owner[k]) { d = k; break; }
+      if(d < 0) break;                                          // retry next tick
+      const size_t hp = (size_t)q->n2 * q->nv * sizeof(float);
+      if(int e = stage_buffer(&S->down[d], &S->down_cap[d], hp * q->n1)) return e;
+      HIP_TRY(hipEventRecord(S->down_done[d], S->stream));     // the run's last step
+      HIP_TRY(hipStreamWaitEvent(S->copy, S->down_done[d], 0));
+      HIP_TRY(hipMemcpy2DAsync(S->down[d], hp, q->in[q->cur], q->s * sizeof(float), hp, q->n1,
+                               hipMemcpyDeviceToHost, S->copy));
+      HIP_TRY(hipMemcpyAsync(S->diags_host + q->diag_slot, S->diags + q->diag_slot,
+                             sizeof(DeviceDiag), hipMemcpyDeviceToHost, S->copy));
+      HIP_TRY(hipEventRecord(S->down_done[d], S->copy));
+      S->down_owner[d] = q;
+      q->copied = S->down_done[d];
+      q->retiring = true;
+    }
+  // release the ones whose copy has completed
+  for(size_t k = 0; k < S->active.size();) {
+    Request *q = S->active[k];
+    if(q->retiring && hipEventQuery(q->copied) == hipSuccess) {
+      int d = 0;
+      while(S->down_owner[d] != q) ++d;
+      memcpy(q->host.data(), S->down[d], q->host.size() * sizeof(float));
+      S->down_owner[d] = nullptr;
+      q->status = (S->diags_host[q->diag_slot].flags & 1u) ? FG2_ERR_NAN : FG2_OK;
+      q->t_done = now;
+      arena_free(S, q->off, q->bytes);
+      S->in_use -= q->bytes;
+      S->free_diag.push_back(q->diag_slot);
+      q->active = false;
+      q->copied = nullptr;
+      S->active.erase(S->active.begin() + k);
+      S->finished.push_back(q);
+    } else {
+      ++k;
+    }
+  }
+  return FG2_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int fg2_server_create(fg2_server **out, size_t hbm_budget, int waves, int tile_h)
+{
+  if(!out || hbm_budget < (1u << 20)) return fail(FG2_ERR_ARG, "bad server arguments");
+  if(waves != 1 && waves != 2 && waves != 4) return fail(FG2_ERR_SHAPE, "waves must be 1, 2 or 4");
+  fg2_server *S = new(std::nothrow) fg2_server;
+  if(!S) return fail(FG2_ERR_NOMEM, "host allocation failed");
+  S->budget = hbm_budget / 256 * 256;
+  S->waves = waves;
+  S->tile_h = tile_h > 0 ? tile_h : 64;
+  S->ndiag = 8192;
+  int dev = 0;
+  hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  if(e == hipSuccess) e = hipGetDevice(&dev);
+  if(e == hipSuccess) e = hipDeviceGetAttribute(&S->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if(e == hipSuccess) e = hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking);
+  for(int k = 0; k < fg2_server::STAGES && e == hipSuccess; ++k) {
+    e = hipEventCreateWithFlags(&S->up_done[k], hipEventDisableTiming);
+    if(e == hipSuccess) e = hipEventCreateWithFlags(&S->down_done[k], hipEventDisableTiming);
+    if(e == hipSuccess) e = hipEventRecord(S->up_done[k], S->copy);
+  }
+  if(e == hipSuccess) e = hipMalloc((void **)&S->arena, S->budget);
+  if(e == hipSuccess) e = hipMalloc((void **)&S->diags, S->ndiag * sizeof(DeviceDiag));
+  if(e == hipSuccess) e = hipHostMalloc((void **)&S->diags_host, S->ndiag * sizeof(DeviceDiag), hipHostMallocDefault);
+  if(e == hipSuccess) e = hipMalloc((void **)&S->counters, fg2_server::SLOTS * sizeof(int));
+  for(int k = 0; k < fg2_server::SLOTS && e == hipSuccess; ++k)
+    e = hipEventCreateWithFlags(&S->rec_done[k], hipEventDisableTiming);
+  for(int k = 0; k < fg2_server::SLOTS && e == hipSuccess; ++k) e = hipEventRecord(S->rec_done[k], S->stream);
+  if(e != hipSuccess) {
+    const int code = e == hipErrorOutOfMemory ? FG2_ERR_NOMEM : FG2_ERR_HIP;
+    fg2_server_free(S);
+    return fail(code, "server setup: %s", hipGetErrorString(e));
+  }
+  S->free_list[0] = S->budget;
+  for(int k = S->ndiag - 1; k >= 0; --k) S->free_diag.push_back(k);
+  *out = S;
+  return FG2_OK;
+}
+
+void fg2_server_free(fg2_server *S)
+{
+  if(!S) return;
+  if(S->stream) (void)hipStreamSynchronize(S->stream);
+  if(S->copy) (void)hipStreamSynchronize(S->copy);
+  for(auto &kv : S->all) delete kv.second;
+  for(int k = 0; k < fg2_server::STAGES; ++k) {
+    if(S->up[k]) (void)hipHostFree(S->up[k]);
+    if(S->down[k]) (void)hipHostFree(S->down[k]);
+    if(S->up_done[k]) (void)hipEventDestroy(S->up_done[k]);
+    if(S->down_done[k]) (void)hipEventDestroy(S->down_done[k]);
+  }
+  if(S->copy) (void)hipStreamDestroy(S->copy);
+  for(int k = 0; k < fg2_server::SLOTS; ++k) {
+    if(S->rec_dev[k]) (void)hipFree(S->rec_dev[k]);
+    if(S->rec_pinned[k]) (void)hipHostFree(S->rec_pinned[k]);
+    if(S->rec_done[k]) (void)hipEventDestroy(S->rec_done[k]);
+  }
+  if(S->arena) (void)hipFree(S->arena);
+  if(S->diags) (void)hipFree(S->diags);
+  if(S->diags_host) (void)hipHostFree(S->diags_host);
+  if(S->counters) (void)hipFree(S->counters);
+  if(S->stream) (void)hipStreamDestroy(S->stream);
+  delete S;
+}
+
+int fg2_server_submit(fg2_server *S, const char *module, int n1, int n2, int steps,
+                      const float *init, double now, long long *id)
+{
+  if(!S || !init || steps < 1) return fail(FG2_ERR_ARG, "bad request");
+  const int m = module_id(module);
+  if(m != HYDRO && m != HYDRO_ISOT) return fail(FG2_ERR_ARG, "served modules: hydro, hydro-isot");
+  if(n1 < HALF || n2 < HALF) return fail(FG2_ERR_SHAPE, "grid %d x %d too small", n1, n2);
+  const size_t need = 2 * (size_t)array_floats(n1, n2, nvar_of(m)) * sizeof(float);
+  if(need > S->budget) return fail(FG2_ERR_NOMEM, "request needs %zu bytes, budget %zu", need, S->budget);
+  Request *q = new(std::nothrow) Request;
+  if(!q) return fail(FG2_ERR_NOMEM, "host allocation failed");
+  q->id = S->next_id++;
+  q->module = m; q->n1 = n1; q->n2 = n2; q->nv = nvar_of(m); q->steps = steps;
+  q->host.assign(init, init + (size_t)n1 * n2 * q->nv);
+  // geometry, parameters and dt of a default run of this module (periodic,
+  // src/global.cpp:28-29 and the module's settings()), no device memory
+  fg2_run tmp;
+  tmp.module = m; tmp.nv = q->nv; tmp.n1 = n1; tmp.n2 = n2;
+  for(auto &p : param_defs(m)) tmp.par.push_back(p.value);
+  fg2_config(&tmp);
+  q->P = make_params(&tmp);
+  q->dt = fg2_getdt(&tmp);
+  q->t_arrive = now;
+  S->queue.push_back(q);
+  S->all[q->id] = q;
+  if(id) *id = q->id;
+  return FG2_OK;
+}
+
+int fg2_server_tick(fg2_server *S, double now, int *active, int *queued, int *finished)
+{
+  if(!S) return fail(FG2_ERR_ARG, "NULL server");
+  if(int e = server_retire(S, now)) return e;
+  if(int e = server_admit(S, now)) return e;
+  if(int e = server_launch(S)) return e;
+  if(int e = server_retire(S, now)) return e;
+  if(S->active.empty() && S->queue.empty()) {
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    HIP_TRY(hipStreamSynchronize(S->copy));
+  }
+  if(active) *active = (int)S->active.size();
+  if(queued) *queued = (int)S->queue.size();
+  if(finished) *finished = (int)S->finished.size();
+  return FG2_OK;
+}
+
+int fg2_server_result(fg2_server *S, long long id, float *out, double *times, int *status)
+{
+  if(!S) return fail(FG2_ERR_ARG, "NULL server");
+  auto it = S->all.find(id);
+  if(it == S->all.end()) return fail(FG2_ERR_ARG, "no request %lld", id);
+  Request *q = it->second;
+  if(times) { times[0] = q->t_arrive; times[1] = q->t_admit; times[2] = q->t_done; }
+  if(status) *status = q->t_done < 0 ? -1 : q->status;
+  if(out && q->t_done >= 0) memcpy(out, q->host.data(), q->host.size() * sizeof(float));
+  return FG2_OK;
+}
+
+int fg2_server_stats(fg2_server *S, long long *cell_steps, size_t *bytes_in_use)
+{
+  if(!S) return fail(FG2_ERR_ARG, "NULL server");
+  if(cell_steps) *cell_steps = S->cell_steps;
+  if(bytes_in_use) *bytes_in_use = S->in_use;
   return FG2_OK;
 }
 

@@ -264,22 +264,26 @@ void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, fl
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int pmod(int a, int n) { const int r = a % n; return r < 0 ? r + n : r; }
 
+template<class M, int NW> struct StepLds {
+  float X[3][M::NV][64 * NW];        // centre row of each stage
+  float G[3][3][M::ND][64 * NW];     // g_k = f(i+k) - f(i-k) of each stage
+};
+
+// One tile of the one-pass step: output rows [i0, i1), output columns
+// [j0, j0 + 64 NW - 18).  All threads of the workgroup call it together.
 template<class M, int NW>
-__global__ __launch_bounds__(64 * NW)
-void step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const long s,
-                 const Params P, const float g0, const float g1, const float g2,
-                 const float b1, const float b2, const int tile_h, const int nstrips,
-                 const int cfl, DeviceDiag *diag)
+__device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *__restrict__ uout,
+                                          const long s, const Params &P, const float g0,
+                                          const float g1, const float g2, const float b1,
+                                          const float b2, const int j0, const int i0,
+                                          const int i1, const int cfl, StepLds<M, NW> &L,
+                                          TileOut &res)
 {
   constexpr int NV = M::NV, ND = M::ND, TW = 64 * NW, H3 = 3 * HALF;
-  __shared__ float X[3][NV][TW];        // centre row of each stage
-  __shared__ float G[3][3][ND][TW];     // g_k = f(i+k) - f(i-k) of each stage
+  auto &X = L.X;
+  auto &G = L.G;
   const int t = threadIdx.x;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int strip = tile % nstrips, chunk = tile / nstrips;
   const int n1 = P.n1, n2 = P.n2;
-  const int j0 = strip * (TW - 2 * H3);
-  const int i0 = chunk * tile_h, i1 = min(n1, i0 + tile_h);
   const int c = j0 - H3 + t;
   const int cc = pmod(c, n2);                         // this lane's column
   const bool out = t >= H3 && t < TW - H3 && c < n2;  // lanes of stage-2 output
@@ -298,7 +302,6 @@ void step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const 
 #pragma unroll
   for(int k = 0; k < 7; ++k) load_row(i0 - H3 + k, W0[k]);
 
-  TileOut res{0, 0.0f, 0.0f};
   const int nit = (i1 - i0) + 4 * HALF;
 
   // one stage on a centre row whose window rows are win(-3..3)
@@ -382,7 +385,55 @@ void step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const 
       }
     }
   }
+}
+
+template<class M, int NW>
+__global__ __launch_bounds__(64 * NW)
+void step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const long s,
+                 const Params P, const float g0, const float g1, const float g2,
+                 const float b1, const float b2, const int tile_h, const int nstrips,
+                 const int cfl, DeviceDiag *diag)
+{
+  __shared__ StepLds<M, NW> L;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int strip = tile % nstrips, chunk = tile / nstrips;
+  const int i0 = chunk * tile_h, i1 = min(P.n1, i0 + tile_h);
+  TileOut res{0, 0.0f, 0.0f};
+  step_tile<M, NW>(uin, uout, s, P, g0, g1, g2, b1, b2, strip * (64 * NW - 6 * HALF), i0, i1,
+                   cfl, L, res);
   commit_tile(res, cfl, diag);
+}
+
+// Batched one-pass step: a persistent grid of workgroups pulls step tiles of
+// many periodic runs from one atomic counter (same step_tile as above, so a
+// run gives the same bits alone and in a batch).
+template<class M, int NW>
+__global__ __launch_bounds__(64 * NW)
+void batch_step_kernel(const BatchRun *__restrict__ runs, const int nruns, const int total_tiles,
+                       int *counter)
+{
+  __shared__ StepLds<M, NW> L;
+  __shared__ int next;
+  for(;;) {
+    __syncthreads();                       // previous tile's LDS reads are done
+    if(threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(next);
+    if(t >= total_tiles) break;
+    int lo = 0, hi = nruns - 1;
+    while(lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if(runs[mid].tile_begin <= t) lo = mid; else hi = mid - 1;
+    }
+    const BatchRun &R = runs[lo];
+    const int lt = t - R.tile_begin;
+    const int strip = lt % R.nstrips, chunk = lt / R.nstrips;
+    const int i0 = chunk * R.tile_h, i1 = min(R.P.n1, i0 + R.tile_h);
+    TileOut res{0, 0.0f, 0.0f};
+    step_tile<M, NW>(R.uin, R.uout, R.s, R.P, R.g[0], R.g[1], R.g[2], (float)(-5.0 / 9.0),
+                     (float)(-153.0 / 128.0), strip * (64 * NW - 6 * HALF), i0, i1, 0, L, res);
+    commit_tile(res, 0, R.diag);
+  }
 }
 
 // Batched substage: a persistent grid of wavefronts pulls tiles of many runs
@@ -742,6 +793,28 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+    }
+  });
+}
+
+hipError_t launch_batch_step(int module, const BatchRun *runs_dev, int nruns, int total_tiles,
+                             int waves, int grid, int *counter, hipStream_t st)
+{
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    if constexpr(M::ID == DISK) {
+      return hipErrorInvalidValue;
+    } else {
+      switch(waves) {
+#define FG2_S(NW)                                                                           \
+      case NW:                                                                              \
+        batch_step_kernel<M, NW><<<grid, 64 * NW, 0, st>>>(runs_dev, nruns, total_tiles, counter); \
+        break;
+      FG2_S(1) FG2_S(2) FG2_S(4)
+#undef FG2_S
+      default: return hipErrorInvalidValue;
+      }
+      return hipGetLastError();
     }
   });
 }

@@ -54,9 +54,14 @@ struct BatchRun {
   long s;
   Params P;
   int tile_h, ntiles, nstrips, tile_begin;  // this run's tiles are [tile_begin, tile_begin+ntiles)
-  float beta, gdt;
+  float beta, gdt;       // substage batches: this stage's beta and gamma dt
+  float g[3];            // step batches: gamma_k dt of the three stages
   DeviceDiag *diag;
 };
+// Batched one-pass step (every run periodic, module hydro or hydro-isot):
+// records hold uin/uout, the tile split and g[3]; one launch per step.
+hipError_t launch_batch_step(int module, const BatchRun *runs_dev, int nruns, int total_tiles,
+                             int waves, int grid, int *counter, hipStream_t st);
 hipError_t launch_batch_substage(int module, int stage, const BatchRun *runs_dev, int nruns,
                                  int total_tiles, int waves, int grid, int *counter,
                                  int cfl, hipStream_t st);

@@ -12,29 +12,39 @@ import numpy as np
 GAMMA = 5.0 / 3.0
 
 
-def gaussian_random_field(n1, n2, kmax, rms, seed, ncomp=1):
+def gaussian_random_field(n1, n2, kmax, rms, seed, ncomp=1, dtype=np.float64):
     """Real random field(s) on a periodic n1 x n2 grid built from every
     integer wavevector 0 < |k| <= kmax with independent complex normal
     amplitudes, scaled so the root-mean-square (over cells, of the vector
-    norm when ncomp > 1) equals rms."""
+    norm when ncomp > 1) equals rms.  Synthesised as Re(E1 A E2^T) with
+    E1[i, a] = exp(2 pi i k_a i / n1), E2[j, b] = exp(2 pi i k_b j / n2) and
+    A the (2 kmax + 1)^2 amplitude matrix: one real matrix product of inner
+    size 2 (2 kmax + 1) per component."""
     rng = np.random.default_rng(seed)
-    k1 = np.fft.fftfreq(n1, 1.0 / n1)[:, None]
-    k2 = np.fft.fftfreq(n2, 1.0 / n2)[None, :]
-    mask = ((k1 ** 2 + k2 ** 2) <= kmax ** 2) & ((k1 != 0) | (k2 != 0))
-    out = []
-    for _ in range(ncomp):
-        spec = (rng.standard_normal((n1, n2)) + 1j * rng.standard_normal((n1, n2))) * mask
-        out.append(np.real(np.fft.ifft2(spec)))
-    f = np.stack(out, axis=-1)
-    norm = np.sqrt(np.mean(np.sum(f ** 2, axis=-1)))
-    return f * (rms / norm if norm > 0 else 0.0)
+    K = int(np.floor(kmax))
+    ks = np.arange(-K, K + 1)
+    mask = (ks[:, None] ** 2 + ks[None, :] ** 2 <= kmax ** 2) & \
+           ((ks[:, None] != 0) | (ks[None, :] != 0))
+    E1 = np.exp(2j * np.pi * np.outer(np.arange(n1), ks) / n1)
+    E2 = np.exp(2j * np.pi * np.outer(np.arange(n2), ks) / n2)
+    right = np.concatenate([E2.real, E2.imag], axis=1).T.astype(dtype)   # 2(2K+1) x n2
+    f = np.empty((n1, n2, ncomp), dtype)
+    for q in range(ncomp):
+        A = (rng.standard_normal(mask.shape) + 1j * rng.standard_normal(mask.shape)) * mask
+        Pm = E1 @ A                                                        # n1 x (2K+1)
+        left = np.concatenate([Pm.real, -Pm.imag], axis=1).astype(dtype)
+        f[..., q] = left @ right
+    norm = np.sqrt(np.mean(np.sum(f.astype(np.float64) ** 2, axis=-1))) if f.size < 2 ** 22 \
+        else np.sqrt(np.mean(np.sum(f ** 2, axis=-1, dtype=np.float64)))
+    f *= (rms / norm if norm > 0 else 0.0)
+    return f
 
 
-def hydro_random(n1, n2, seed=1, kmax=4, rms=0.01, gamma=GAMMA):
+def hydro_random(n1, n2, seed=1, kmax=4, rms=0.01, gamma=GAMMA, dtype=np.float64):
     """hydro: rho = 1, p = 1/gamma (so c_s = 1), velocity a Gaussian random
     field with |k| <= kmax and rms |u| = rms * c_s.  Returns (n1, n2, 4)."""
-    u = gaussian_random_field(n1, n2, kmax, rms, seed, ncomp=2)
-    x = np.zeros((n1, n2, 4))
+    u = gaussian_random_field(n1, n2, kmax, rms, seed, ncomp=2, dtype=dtype)
+    x = np.zeros((n1, n2, 4), dtype)
     x[..., 1:3] = u
     x[..., 3] = np.log((1.0 / gamma) / (gamma - 1.0))   # e = p / (rho (gamma - 1))
     return x
@@ -56,11 +66,11 @@ def hydro_kh(n1, n2, seed=2, amp=0.01, gamma=GAMMA, l1=1.0, l2=1.0):
     return x
 
 
-def isot_density(n1, n2, seed=3, kmax=8, rms=0.05):
+def isot_density(n1, n2, seed=3, kmax=8, rms=0.05, dtype=np.float64):
     """hydro-isot: lnd = ln(1 + delta), delta a Gaussian random field with
     |k| <= kmax and rms 0.05, u = 0.  Returns (n1, n2, 3)."""
-    d = gaussian_random_field(n1, n2, kmax, rms, seed)[..., 0]
-    x = np.zeros((n1, n2, 3))
+    d = gaussian_random_field(n1, n2, kmax, rms, seed, dtype=dtype)[..., 0]
+    x = np.zeros((n1, n2, 3), dtype)
     x[..., 0] = np.log1p(d)
     return x
 

@@ -134,6 +134,11 @@ int fg2_set_dt_mode(fg2_run *run, int mode);
  * automatic choice (or the autotuned one, see fg2_autotune). */
 int fg2_set_tile(fg2_run *run, int tile_h, int waves);
 int fg2_get_tile(const fg2_run *run, int *tile_h, int *waves);
+/* Time the run's fused kernel on its own state for candidate tile shapes
+ * and keep the fastest (cached per module and grid size for the process);
+ * replaces the block-size heuristic of setup() (src/setup.cpp:39-58).  The
+ * state is unchanged; *tile_h, *waves (may be NULL) receive the choice. */
+int fg2_autotune(fg2_run *run, int *tile_h, int *waves);
 
 /* One low-storage RK3 step of size dt at time t: first_order, then three
  * substages; synchronises the run's stream and checks the NaN flag.
@@ -184,6 +189,37 @@ int fg2_batch_step(fg2_batch *batch, const double *dts);
  * replay it `replays` times. */
 int fg2_batch_graph_run(fg2_batch *batch, const double *dts, int steps, int replays);
 void *fg2_batch_stream(fg2_batch *batch);
+/* Choose the batch's tile height and workgroup shape by timing batched
+ * steps of size dts on the runs' states (states unchanged). */
+int fg2_batch_autotune(fg2_batch *batch, const double *dts, int *tile_h, int *waves);
+
+/* ---- serving: continuous batching of runs of different sizes ---- */
+
+typedef struct fg2_server fg2_server;
+
+/* A server that keeps admitted runs' state on the current device in an
+ * arena of hbm_budget bytes (allocated once; a run holds two state arrays)
+ * and advances all active runs together each tick (one batched one-pass
+ * launch per module; served runs are periodic hydro or hydro-isot with the
+ * module's default parameters).  waves, tile_h: the batched kernel's
+ * workgroup and tile. */
+int fg2_server_create(fg2_server **server, size_t hbm_budget, int waves, int tile_h);
+void fg2_server_free(fg2_server *server);
+/* Queue a request of `steps` RK3 steps (closed-form dt per step) from the
+ * initial state init (n1*n2*NVAR floats, .raw order, copied); `now` is the
+ * caller's clock, used for the latency record. */
+int fg2_server_submit(fg2_server *server, const char *module, int n1, int n2, int steps,
+                      const float *init, double now, long long *id);
+/* Retire finished runs, admit queued requests that fit the budget, and
+ * launch one step of every active run.  Counts of active, queued and
+ * finished requests are returned (pointers may be NULL). */
+int fg2_server_tick(fg2_server *server, double now, int *active, int *queued, int *finished);
+/* times[3] = arrival, admission, completion (caller's clock; -1 if not yet);
+ * *status = -1 while running, else FG2_OK / FG2_ERR_NAN; out (may be NULL)
+ * receives the final state once finished. */
+int fg2_server_result(fg2_server *server, long long id, float *out, double *times, int *status);
+/* Total cell-updates launched so far and the device bytes held by runs. */
+int fg2_server_stats(fg2_server *server, long long *cell_steps, size_t *bytes_in_use);
 
 #ifdef __cplusplus
 }

@@ -193,3 +193,46 @@ def test_step_kernel_equals_substages(fg2, module, n1, n2):
         r = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, tile=tile)
         assert r.run(0.0, dt, 3) == 0
         np.testing.assert_array_equal(r.download(), y)
+
+
+def test_autotune_keeps_state_and_bits(fg2):
+    """The autotuner times kernels on the run's state without changing it,
+    and the tuned shape gives the same bits as the automatic one."""
+    n1, n2 = 200, 300
+    x = state("hydro", n1, n2, seed=31)
+    a = _run(fg2, "hydro", n1, n2, x, fg2.PATH_FUSED)
+    b = _run(fg2, "hydro", n1, n2, x, fg2.PATH_FUSED)
+    th, w = b.autotune()
+    _say("autotuned", th, w)
+    np.testing.assert_array_equal(b.download(), x.astype(np.float32))
+    dt = 0.25 * a.getdt()
+    assert a.run(0.0, dt, 3) == 0 and b.run(0.0, dt, 3) == 0
+    np.testing.assert_array_equal(a.download(), b.download())
+
+
+def test_server_alone_vs_mix(fg2):
+    """Continuous batching: requests of both modules and different sizes and
+    step counts, admitted under a tight HBM budget while others run, give
+    the same bits as each run stepped alone."""
+    import time
+    reqs = [("hydro", 64, 96, 7), ("hydro-isot", 128, 64, 12), ("hydro", 100, 70, 3),
+            ("hydro-isot", 70, 130, 9), ("hydro", 256, 256, 5), ("hydro-isot", 96, 96, 1)]
+    xs = [state(m, a, b, seed=40 + k) for k, (m, a, b, _) in enumerate(reqs)]
+    budget = 3 * 4 * 4 * 300 * 300 + (1 << 16)     # forces queueing
+    srv = fg2.Server(budget, waves=4, tile_h=48)
+    ids = []
+    t0 = time.perf_counter()
+    for k, ((m, a, b, n), x) in enumerate(zip(reqs, xs)):
+        ids.append(srv.submit(m, x, n, time.perf_counter() - t0))
+        srv.tick(time.perf_counter() - t0)           # arrivals interleave with ticks
+    for _ in range(200):
+        act, que, fin = srv.tick(time.perf_counter() - t0)
+        if fin == len(reqs) and act == 0:
+            break
+    assert fin == len(reqs)
+    for (m, a, b, n), x, i in zip(reqs, xs, ids):
+        st, times, y = srv.result(i)
+        assert st == 0 and times[2] >= times[1] >= times[0]
+        r = _run(fg2, m, a, b, x, fg2.PATH_FUSED)
+        assert r.run(0.0, r.getdt(), n) == 0
+        np.testing.assert_array_equal(y, r.download())

@@ -236,3 +236,17 @@ def test_server_alone_vs_mix(fg2):
         r = _run(fg2, m, a, b, x, fg2.PATH_FUSED)
         assert r.run(0.0, r.getdt(), n) == 0
         np.testing.assert_array_equal(y, r.download())
+
+
+def test_disk_precision_small(fg2):
+    """Precision study at reduced size (the full 2048^2 x 2000-step study is
+    tests/precision_study.py): FP32 GPU vs FP64 oracle, disk 192^2, 200
+    steps at 0.9 CFL; per-field relative L2 below 1e-4 and mass and angular
+    momentum drift of the two sides within 1e-5 of each other."""
+    import precision_study as ps
+    r = ps.run(192, 200, progress=False)
+    _say(r)
+    assert r["gpu_status"] == 0 and r["oracle_status"] == 0
+    assert max(r["rel_l2"].values()) < 1e-4, r["rel_l2"]
+    for key in ("mass_drift", "angular_momentum_drift"):
+        assert abs(r[key]["gpu_fp32"] - r[key]["oracle_fp64"]) < 1e-5, r[key]

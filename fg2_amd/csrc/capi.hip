@@ -159,6 +159,7 @@ Params make_params(const fg2_run *r)
     P.b1[k] = (float)(c2[k] / (D1 * D1));
     P.b2[k] = (float)(c2[k] / (D2 * D2));
   }
+  P.blap0 = (float)(c2[0] / (D1 * D1) + c2[0] / (D2 * D2));
   for(size_t k = 0; k < r->par.size() && k < 16; ++k) P.par[k] = (float)r->par[k];
   P.rand1 = r->rand1; P.rand2 = r->rand2;
   return P;

@@ -54,6 +54,8 @@ struct Params {
   float b1[4], b2[4];
   // D12: m[k] * mix_k
   float m12[3];
+  // D11 + D22 centre coefficient, b1[0] + b2[0]
+  float blap0;
   // module parameters, in the module's own order (see physics.h)
   float par[16];
   // hydro-isot first_order random phases (src/hydro-isot/Euler.cuh:24-25)

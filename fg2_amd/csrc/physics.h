@@ -1,7 +1,7 @@
 // physics.h -- the three physics modules on the device (FP32).
 //
-// Every kernel (plain kick, fused substage, batched substage) evaluates the
-// right-hand side through eval_rhs<M>() below from the same stencil
+// Every kernel (plain kick, fused substage, one-pass step, batched) evaluates
+// the right-hand side through M::rhs() below from the same stencil
 // primitives, with explicit FMAs in the operators.  The per-cell instruction
 // sequence therefore does not depend on which kernel, tile shape or batch a
 // cell is computed in, and the paths agree bit for bit.
@@ -44,6 +44,17 @@ template<class Pr> __device__ __forceinline__ float D2 (const Params &P, const P
 template<class Pr> __device__ __forceinline__ float D11(const Params &P, const Pr &x, int q) { return op_dd(P.b1, x.c[q], x.sv[0][q], x.sv[1][q], x.sv[2][q]); }
 template<class Pr> __device__ __forceinline__ float D22(const Params &P, const Pr &x, int q) { return op_dd(P.b2, x.c[q], x.sh[0][q], x.sh[1][q], x.sh[2][q]); }
 template<class Pr> __device__ __forceinline__ float D12(const Params &P, const Pr &x, int d) { return op_d(P.m12, x.mx[0][d], x.mx[1][d], x.mx[2][d]); }
+// D11 + D22 in one chain: (b1[0] + b2[0]) f0 + sum_k b1[k] sv_k + b2[k] sh_k
+template<class Pr> __device__ __forceinline__ float LAP(const Params &P, const Pr &x, int q)
+{
+  float a = P.blap0 * x.c[q];
+#pragma unroll
+  for(int k = 0; k < 3; ++k) {
+    a = __builtin_fmaf(P.b1[k + 1], x.sv[k][q], a);
+    a = __builtin_fmaf(P.b2[k + 1], x.sh[k][q], a);
+  }
+  return a;
+}
 
 constexpr float THIRD = 1.0f / 3.0f;
 
@@ -67,7 +78,7 @@ struct Hydro {
     for(int q = 0; q < NV; ++q) dt[q] = -(u1 * d1[q] + u2 * d2[q]);     // advection
 
     const float gamma1 = gamma - 1.0f;                                     // pressure
-    const float temp   = gamma1 * expf(x.c[3]);
+    const float temp   = gamma1 * __expf(x.c[3]);
     const float div_u  = d1[1] + d2[2];
     dt[0] -= div_u;
     dt[1] -= temp * (d1[0] + d1[3]);
@@ -80,7 +91,7 @@ struct Hydro {
     const float two_nus = 2.0f * nus;
     dt[1] += two_nus * (s11 * d1[0] + s12 * d2[0]);
     dt[2] += two_nus * (s12 * d1[0] + s22 * d2[0]);
-    dt[3] += (gamma1 / temp) *
+    dt[3] += (gamma1 * __builtin_amdgcn_rcpf(temp)) *
       (two_nus * (s11 * s11 + 2.0f * s12 * s12 + s22 * s22) + nub * div_u * div_u);
 
     const float d11_u1 = D11(P, x, 1), d11_u2 = D11(P, x, 2);             // viscosity
@@ -90,11 +101,14 @@ struct Hydro {
     dt[1] += nus * (d11_u1 + d22_u1) + mixed * (d11_u1 + d12_u2);
     dt[2] += nus * (d11_u2 + d22_u2) + mixed * (d12_u1 + d22_u2);
 
-    const float d_lnd_2 = d1[0] * d1[0] + d2[0] * d2[0];                  // diffusion
-    const float d_lne_2 = d1[3] * d1[3] + d2[3] * d2[3];
+    const float d_lne_2 = d1[3] * d1[3] + d2[3] * d2[3];                  // diffusion
     const float ed      = gamma1 * kappa;
-    dt[0] += dd * (D11(P, x, 0) + D22(P, x, 0) + d_lnd_2);
-    dt[3] += ed * (D11(P, x, 3) + D22(P, x, 3) + d_lne_2);
+    // dd = 0 (the default) contributes 0 * (finite) = 0: skip the work
+    if(dd != 0.0f) {
+      const float d_lnd_2 = d1[0] * d1[0] + d2[0] * d2[0];
+      dt[0] += dd * (LAP(P, x, 0) + d_lnd_2);
+    }
+    dt[3] += ed * (LAP(P, x, 3) + d_lne_2);
   }
 
   static __device__ __forceinline__ void first_order(const Params &, float *, float, int, int) {}
@@ -152,7 +166,8 @@ struct HydroIsot {
     dt[1] += nus * (d11_u1 + d22_u1) + mixed * (d11_u1 + d12_u2);
     dt[2] += nus * (d11_u2 + d22_u2) + mixed * (d12_u1 + d22_u2);
 
-    dt[0] += dd * (D11(P, x, 0) + d1[0] * d1[0] + D22(P, x, 0) + d2[0] * d2[0]);
+    if(dd != 0.0f)                         // dd = 0 (the default) adds 0
+      dt[0] += dd * (LAP(P, x, 0) + d1[0] * d1[0] + d2[0] * d2[0]);
 
     if(ff * kk < 0.0f) dt[2] += ff * cosf(kk * (float)i * P.Delta1);     // Kolmogorov forcing
   }

@@ -32,7 +32,6 @@
 #include <vector>
 #include <algorithm>
 #include <limits>
-#include <xmmintrin.h>
 
 namespace {
 
@@ -794,18 +793,6 @@ void orc_deriv(void *p, int op, double *out)
 
 int orc_step(void *p, double t, double dt)
 { return dispatch((OrcHandle *)p, [&](auto &S) { return step(S, t, dt); }); }
-
-// Flush FP32 denormals to zero on every OpenMP thread (x86 MXCSR FTZ|DAZ),
-// for the FP32 CPU baseline: small-amplitude states otherwise run through
-// denormal micro-code.  The FP64 oracle never calls it.
-void orc_flush_denormals(int on)
-{
-  #pragma omp parallel
-  {
-    const unsigned m = _mm_getcsr();
-    _mm_setcsr(on ? (m | 0x8040u) : (m & ~0x8040u));
-  }
-}
 
 // n steps of fixed dt; returns the NaN flag.
 int orc_run(void *p, double t, double dt, int n)

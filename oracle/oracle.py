@@ -93,7 +93,6 @@ def lib():
             "orc_getdt": (D, [P]),
             "orc_getdt_cfl": (D, [P]),
             "orc_nan_flag": (I, [P]),
-            "orc_flush_denormals": (None, [I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -222,7 +221,3 @@ class Oracle:
     def nan_flag(self):
         return lib().orc_nan_flag(self.h)
 
-
-def flush_denormals(on=True):
-    """FTZ/DAZ on the oracle's OpenMP threads (FP32 CPU baseline only)."""
-    lib().orc_flush_denormals(1 if on else 0)

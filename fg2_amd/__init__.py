@@ -11,6 +11,11 @@ simulation in `namespace global`; here a `Run` holds it:
     step(t, dt)             -> Run.step(t, dt)            src/step.cpp:22
     solve(t, t_final, ...)  -> Run.solve(t, t_final)      src/solve.cpp:25
     Euler/bcond/kick/drift  -> Run.euler/bcond/kick/drift src/fg2.h:81-84
+    init(name)              -> Run.init(name, seed)       src/init.cpp:26
+    load(name) / dump(...)  -> Run.load(path) / Run.dump  src/io.cpp:24,56
+
+`fg2_amd.raw` reads and writes .raw snapshots on the host, and the
+command-line driver `fg2_amd/fg2run` mirrors the reference's main().
 
 All arithmetic runs in the HIP library; this module only moves host arrays
 and calls the C ABI.  The library is opened on first use, which raises if
@@ -91,6 +96,9 @@ def _load():
         "fg2_solve": (I, [P, DP, D, IP]),
         "fg2_sums": (I, [P, DP]),
         "fg2_perf_model": (I, [P, DP, DP, DP]),
+        "fg2_init": (I, [P, ctypes.c_char_p, ctypes.c_ulonglong]),
+        "fg2_dump": (I, [P, ctypes.c_char_p, D]),
+        "fg2_load": (I, [P, ctypes.c_char_p, DP]),
         "fg2_batch_create": (I, [PP, PP, I, I, I]),
         "fg2_batch_free": (None, [P]),
         "fg2_batch_step_async": (I, [P, DP]),
@@ -257,6 +265,23 @@ class Run:
         x = np.empty(self.shape, np.float32)
         _check(lib.fg2_download_v(self.h, _fp(x)))
         return x
+
+    # -- initial conditions and snapshots ----------------------------------
+    def init(self, name, seed=0):
+        """Initial condition `name` evaluated on the device (init(),
+        src/init.cpp:26-42); KH and Kf draw from SplitMix64(seed)."""
+        _check(lib.fg2_init(self.h, name.encode(), int(seed)))
+
+    def dump(self, path, time):
+        """Write a .raw snapshot (dump(), src/io.cpp:56-76)."""
+        _check(lib.fg2_dump(self.h, os.fsencode(path), float(time)))
+
+    def load(self, path):
+        """Read a .raw snapshot, taking l1, l2, p1, p2, c from it, and
+        return its time (load(), src/io.cpp:24-54).  Call config() after."""
+        t = ctypes.c_double(0.0)
+        _check(lib.fg2_load(self.h, os.fsencode(path), ctypes.byref(t)))
+        return t.value
 
     # -- device operations (src/fg2.h:81-84) ------------------------------
     def euler(self, dt):

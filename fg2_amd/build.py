@@ -17,7 +17,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfg2_amd.so")
 COPY_EXE = os.path.join(HERE, "hbm_copy")
-SOURCES = ["kernels.hip", "capi.hip"]
+DRIVER_EXE = os.path.join(HERE, "fg2run")
+SOURCES = ["kernels.hip", "capi.hip", "ics.hip"]
 HEADERS = ["fg2_layout.h", "physics.h", "launch.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -58,6 +59,17 @@ def build(verbose=False, jobs=2):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    # the command-line driver, linked against the library (rpath $ORIGIN)
+    # (host code only: g++ against the HIP runtime API)
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
+           os.path.join(CSRC, "driver.cpp"), "-o", DRIVER_EXE,
+           "-L", HERE, "-lfg2_amd", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
     return LIB
 
 

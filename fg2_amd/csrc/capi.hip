@@ -458,6 +458,78 @@ int fg2_upload(fg2_run *r, const float *h)
   return copy_interior(r, r->in[r->cur], (float *)h, true);
 }
 int fg2_download(fg2_run *r, float *h) { return r ? copy_interior(r, r->in[r->cur], h, false) : fail(FG2_ERR_ARG, "NULL run"); }
+
+int fg2_init(fg2_run *r, const char *name, unsigned long long seed)
+{
+  if(int e = check_run(r)) return e;
+  const std::string n = name ? name : "";
+  int ic = IC_ZEROS;                                 // the modules' pick() (init.h)
+  if(r->module == HYDRO) {
+    if(n == "blast") ic = IC_BLAST;
+    else if(n == "bow") ic = IC_BOW;
+    else if(n == "implos") ic = IC_IMPLOS;
+    else if(n == "KH") ic = IC_KH;
+    else if(n == "Sod") ic = IC_SOD;
+  } else if(r->module == HYDRO_ISOT) {
+    if(n == "Kf") ic = IC_KF;
+  } else {
+    ic = IC_HAWLEY;
+  }
+  const Params P = make_params(r);
+  HIP_TRY(launch_ic(r->module, ic, r->in[r->cur], r->s, r->n1, r->n2, (float)r->l1, (float)r->l2,
+                    P.par, seed, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  r->lam_valid = false;
+  return FG2_OK;
+}
+
+// .raw snapshot header (src/io.cpp:62-67): int32 size[6], double time, R info[3].
+int fg2_dump(fg2_run *r, const char *path, double time)
+{
+  if(!r || !path) return fail(FG2_ERR_ARG, "NULL argument");
+  std::vector<float> x((size_t)r->n1 * r->n2 * r->nv);
+  if(int e = copy_interior(r, r->in[r->cur], x.data(), false)) return e;
+  FILE *f = fopen(path, "wb");
+  if(!f) return fail(FG2_ERR_ARG, "cannot open \"%s\" for writing", path);
+  const int32_t size[6] = {r->n1, r->n2, r->nv, (int32_t)sizeof(float), r->p1, r->p2};
+  const float info[3] = {(float)r->l1, (float)r->l2, (float)r->c};
+  bool ok = fwrite(size, sizeof size, 1, f) == 1 && fwrite(&time, sizeof time, 1, f) == 1 &&
+            fwrite(info, sizeof info, 1, f) == 1 &&
+            fwrite(x.data(), sizeof(float), x.size(), f) == x.size();
+  ok = (fclose(f) == 0) && ok;
+  return ok ? FG2_OK : fail(FG2_ERR_ARG, "short write to \"%s\"", path);
+}
+
+int fg2_load(fg2_run *r, const char *path, double *time)
+{
+  if(!r || !path) return fail(FG2_ERR_ARG, "NULL argument");
+  FILE *f = fopen(path, "rb");
+  if(!f) return fail(FG2_ERR_ARG, "cannot open \"%s\"", path);
+  int32_t size[6];
+  double t = 0;
+  float info[3];
+  if(fread(size, sizeof size, 1, f) != 1 || fread(&t, sizeof t, 1, f) != 1 ||
+     fread(info, sizeof info, 1, f) != 1) {
+    fclose(f);
+    return fail(FG2_ERR_ARG, "\"%s\": short header", path);
+  }
+  if(size[0] != r->n1 || size[1] != r->n2 || size[2] != r->nv || size[3] != (int32_t)sizeof(float)) {
+    fclose(f);
+    return fail(FG2_ERR_SHAPE, "input data is not compatible with the setup "
+                "(file %d x %d x %d of %d-byte words, run %d x %d x %d)",
+                size[0], size[1], size[2], size[3], r->n1, r->n2, r->nv);
+  }
+  std::vector<float> x((size_t)r->n1 * r->n2 * r->nv);
+  const bool ok = fread(x.data(), sizeof(float), x.size(), f) == x.size();
+  fclose(f);
+  if(!ok) return fail(FG2_ERR_ARG, "\"%s\": short data", path);
+  if(!(info[0] > 0) || !(info[1] > 0) || !(info[2] > 0))
+    return fail(FG2_ERR_ARG, "\"%s\": l1, l2, c must be > 0", path);
+  if(int e = fg2_upload(r, x.data())) return e;
+  fg2_set_geometry(r, info[0], info[1], size[4], size[5], info[2]);   // src/io.cpp:49-53
+  if(time) *time = t;
+  return FG2_OK;
+}
 int fg2_upload_v(fg2_run *r, const float *h) { return r ? copy_interior(r, r->in[2], (float *)h, true) : fail(FG2_ERR_ARG, "NULL run"); }
 int fg2_download_v(fg2_run *r, float *h) { return r ? copy_interior(r, r->in[2], h, false) : fail(FG2_ERR_ARG, "NULL run"); }
 

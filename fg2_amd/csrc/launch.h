@@ -48,6 +48,12 @@ hipError_t launch_cfl(int module, const float *u, long s, const Params &P, Devic
 hipError_t launch_sums(int module, const float *u, long s, const Params &P, double *partial,
                        double *out, hipStream_t st);
 
+// Initial conditions on the device (ics.hip); par = the run's 16 FP32
+// module parameters.  ic selects the function of the module's init.h.
+enum IcId { IC_ZEROS = 0, IC_BLAST, IC_BOW, IC_IMPLOS, IC_KH, IC_SOD, IC_KF, IC_HAWLEY };
+hipError_t launch_ic(int module, int ic, float *u, long s, int n1, int n2, float l1, float l2,
+                     const float *par, unsigned long long seed, hipStream_t st);
+
 // Batched path: many independent runs in one persistent launch.
 struct BatchRun {
   const float *uin; float *uout; float *v;

@@ -102,6 +102,30 @@ int fg2_download(fg2_run *run, float *host);
 int fg2_upload_v(fg2_run *run, const float *host);
 int fg2_download_v(fg2_run *run, float *host);
 
+/* ---- initial conditions and .raw snapshots ---- */
+
+/* Fill the state with the module's initial condition `name`, evaluated on
+ * the device at the cell centres; replaces init(name) (src/fg2.h:70,
+ * src/init.cpp:26-42, src/{hydro,hydro-isot,disk}/init.h).  Requires
+ * fg2_config.  hydro: "blast", "bow", "implos", "KH", "Sod", anything else
+ * zeros; hydro-isot: "Kf", anything else zeros; disk: the Hawley torus
+ * whatever the name.  The rand() draws of KH and Kf come from a SplitMix64
+ * stream seeded with `seed` (draw q = output q+1 from state seed, in the
+ * reference's row-major draw order), mapped to ((z >> 11) + 1/2) 2^-53. */
+int fg2_init(fg2_run *run, const char *name, unsigned long long seed);
+
+/* Write the state as a .raw snapshot: int32[6] {n1, n2, NVAR, 4, p1, p2},
+ * double time, float[3] {l1, l2, c}, then n1*n2*NVAR floats in row-major
+ * AoS order; replaces dump(name, time) (src/fg2.h:72, src/io.cpp:56-76). */
+int fg2_dump(fg2_run *run, const char *path, double time);
+
+/* Read a .raw snapshot into the state and take p1, p2, l1, l2, c from it;
+ * *time (may be NULL) receives its time.  A header whose n1, n2, NVAR or
+ * word size differs from the run's is FG2_ERR_SHAPE ("input data is not
+ * compatible with the setup"); replaces load(name) (src/fg2.h:71,
+ * src/io.cpp:24-54).  As in the reference, call fg2_config afterwards. */
+int fg2_load(fg2_run *run, const char *path, double *time);
+
 /* ---- device operations, 1:1 with src/fg2.h:81-84 (asynchronous) ---- */
 
 /* Point-wise first_order pre-step; replaces Euler(u, dt) (src/kick.cu:128-138). */

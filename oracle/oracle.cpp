@@ -20,6 +20,8 @@
 //   - closed-form time step                               (src/getdt.cpp:22-29)
 //   - Delta = l/n, module settings()                      (src/config.cpp:24-41,
 //                                                          src/*/config.h)
+//   - initial conditions at the cell centres              (src/init.cpp:26-42,
+//                                                          src/*/init.h)
 // and a CFL time step with advective and diffusive limits, which the
 // reference only has as a TODO (src/getdt.cpp:26-28).
 //
@@ -607,6 +609,110 @@ template<class R> double getdt_cfl(const Sim<R> &S)
   return S.c * dt;
 }
 
+// ---------------------------------------------------------------------------
+// Initial conditions, src/init.cpp:26-42 and the modules' init.h.  The
+// reference's default build has R = float: the cell centres and the
+// parameters its IC functions read are FP32 values, so they are rounded to
+// float here too; each function is then evaluated in FP64.  The libc rand()
+// draws are replaced by one sequential SplitMix64 stream started at `seed`,
+// consumed in the reference's call order (row-major over i then j; one draw
+// per cell in KH, two in Kf), each mapped to ((z >> 11) + 1/2) 2^-53.
+// ---------------------------------------------------------------------------
+double open_unit(uint64_t &state) { return ((double)(splitmix64(state) >> 11) + 0.5) * 0x1.0p-53; }
+double fermi_dirac(double a, double d) { return 1.0 / (std::exp(a / d) + 1.0); }
+
+// src/hydro/init.h:26-112 -> (lnd, u1, u2, lne)
+void ic_hydro(const std::string &name, double g, double l1, double l2, double x, double y,
+              uint64_t &rng, double *c)
+{
+  x = (float)(x - 0.5 * l1);                       // R-typed shift, init.h:30-31 (each IC)
+  y = (float)(y - 0.5 * l2);
+  double lnd = 0, u1 = 0, lne = 0;
+  if(name == "blast") {                            // init.h:28-36
+    lne = std::log(((x * x + y * y < 0.01) ? 1.0 : 0.01) / (g - 1.0));
+  } else if(name == "bow") {                       // init.h:38-49
+    const double r = std::sqrt(x * x + y * y);
+    const double d = 9.9 * fermi_dirac(r - 0.01, 0.001) + 0.1;
+    lnd = std::log(d);
+    u1 = (10.0 / d - 1.0) / 99.0;
+    lne = std::log(0.01 / d / (g - 1.0));
+  } else if(name == "implos") {                    // init.h:51-60
+    const double d = 0.125 + 0.875 * fermi_dirac(0.15 - std::fabs(x) - std::fabs(y), 0.001);
+    const double p = 0.14 + 0.86 * fermi_dirac(0.15 - std::fabs(x) - std::fabs(y), 0.001);
+    lnd = std::log(d);
+    lne = std::log(p / d / (g - 1.0));
+  } else if(name == "KH") {                        // init.h:62-79
+    double d;
+    if(std::fabs(y) < 0.25) { d = 2.0; u1 = 0.5 + 0.01 * open_unit(rng) - 0.005; }
+    else                    { d = 1.0; u1 = -0.5 + 0.01 * open_unit(rng) - 0.005; }
+    lnd = std::log(d);
+    lne = std::log(2.5 / d / (g - 1.0));
+  } else if(name == "Sod") {                       // init.h:81-97
+    double d, P;
+    if(std::fmod(x - 2 * y + 1.5, 1.0) < 0.5) { d = 1.0; P = 1.0; }
+    else                                      { d = 0.125; P = 0.1; }
+    lnd = std::log(d);
+    lne = std::log(P / d / (g - 1.0));
+  }                                                // otherwise zeros, init.h:99-102
+  c[0] = lnd; c[1] = u1; c[2] = 0.0; c[3] = lne;
+}
+
+// src/hydro-isot/init.h:21-49 -> (lnd, u1, u2)
+void ic_isot(const std::string &name, double nus, double fi, double ki, double x,
+             uint64_t &rng, double *c)
+{
+  c[0] = c[1] = c[2] = 0.0;
+  if(name != "Kf") return;
+  const double r1 = open_unit(rng);
+  const double r2 = open_unit(rng);
+  const double q1 = std::sqrt(-2.0 * std::log(r1)) * std::sin(2.0 * M_PI * r2);
+  const double q2 = std::sqrt(-2.0 * std::log(r1)) * std::cos(2.0 * M_PI * r2);
+  const double a = fi / (nus * ki * ki);
+  c[1] = 1.0e-9 * q1;
+  c[2] = 1.0e-9 * q2 + a * std::cos(ki * x);
+}
+
+// src/disk/init.h:25-68 (Hawley 2000 torus) -> (lnd, ur, uz, Omg, lne, lner)
+void ic_disk(double M, double rin, double Gamma, double ar, double lnr, double theta, double *c)
+{
+  const double r = rin * std::exp(lnr);
+  const double cyl_r = r * std::sin(theta);
+  const double q0 = 2.0, r0 = 16.0, d0 = 1.0, e0 = 0.01, bg = 0.0001;
+  const double g1 = Gamma - 1.0;
+  const double q1 = 2.0 * q0 - 2.0;
+  const double lK = std::sqrt(M * std::pow(r0, q1 - 1.0));
+  const double K = (g1 * e0) / std::pow(d0, g1);
+  double den = Gamma * e0 - M / r0 + lK * lK / (std::pow(r0, q1) * q1)
+             + M / r - lK * lK / (std::pow(cyl_r, q1) * q1);
+  if(den > 0.0) den = std::pow(den * g1 / (Gamma * K), 1.0 / g1);
+  else          den = -std::pow(-den * g1 / (Gamma * K), 1.0 / g1);
+  double Omg = lK * std::pow(cyl_r, -q0);
+  if(den < bg) { den = bg; Omg *= std::sin(theta) * std::sin(theta); }
+  const double eng = K * std::pow(den, g1) / g1;
+  const double tbg = K * std::pow(bg, g1);
+  const double rad = ar * tbg * tbg * tbg * tbg / bg;
+  c[0] = std::log(den); c[1] = 0.0; c[2] = 0.0; c[3] = Omg; c[4] = std::log(eng); c[5] = std::log(rad);
+}
+
+template<class R> void init_ic(Sim<R> &S, const std::string &name, uint64_t seed)
+{
+  uint64_t rng = seed;
+  const float l1 = (float)S.l1, l2 = (float)S.l2;
+  auto pf = [&](const char *n) { return (double)(float)S.get(n); };
+  for(long i = 0; i < S.n1; ++i) {
+    const float x = (float)((double)l1 * (i + 0.5) / S.n1);   // src/init.cpp:31
+    for(long j = 0; j < S.n2; ++j) {
+      const float y = (float)((double)l2 * (j + 0.5) / S.n2); // src/init.cpp:33
+      double c[6];
+      if(S.module == HYDRO) ic_hydro(name, pf("gamma"), l1, l2, x, y, rng, c);
+      else if(S.module == HYDRO_ISOT) ic_isot(name, pf("nus"), pf("f"), pf("k"), x, rng, c);
+      else ic_disk(pf("M"), pf("rin"), pf("gamma"), pf("ar"), x, y, c);
+      R *u = S.cell(S.u, i, j);
+      for(int q = 0; q < S.nv; ++q) u[q] = (R)c[q];
+    }
+  }
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -666,6 +772,12 @@ void orc_get_geometry(void *p, double *out)  // l1 l2 p1 p2 c Delta1 Delta2
 }
 
 void orc_config(void *p) { dispatch((OrcHandle *)p, [](auto &S) { config(S); return 0; }); }
+
+void orc_init(void *p, const char *name, unsigned long long seed)
+{ dispatch((OrcHandle *)p, [&](auto &S) { init_ic(S, name ? name : "", seed); return 0; }); }
+
+unsigned long long orc_splitmix64(unsigned long long *state)
+{ uint64_t s = *state; const uint64_t z = splitmix64(s); *state = s; return z; }
 
 void orc_seed(void *p, unsigned long long seed)
 { dispatch((OrcHandle *)p, [&](auto &S) { S.rng = seed; return 0; }); }

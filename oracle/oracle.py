@@ -93,6 +93,8 @@ def lib():
             "orc_getdt": (D, [P]),
             "orc_getdt_cfl": (D, [P]),
             "orc_nan_flag": (I, [P]),
+            "orc_init": (None, [P, ctypes.c_char_p, ctypes.c_ulonglong]),
+            "orc_splitmix64": (ctypes.c_ulonglong, [ctypes.POINTER(ctypes.c_ulonglong)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -152,6 +154,11 @@ class Oracle:
 
     def config(self):
         lib().orc_config(self.h)
+
+    def init(self, name, seed=0):
+        """Initial condition `name` at the cell centres (the reference's
+        init(), src/init.cpp:26-42), rand() replaced by SplitMix64(seed)."""
+        lib().orc_init(self.h, name.encode(), int(seed))
 
     def rand(self):
         out = np.zeros(2)
@@ -221,3 +228,9 @@ class Oracle:
     def nan_flag(self):
         return lib().orc_nan_flag(self.h)
 
+
+def splitmix64(state, count):
+    """The first `count` outputs of SplitMix64 started at `state` (the
+    oracle's stand-in for libc rand())."""
+    st = ctypes.c_ulonglong(state)
+    return [lib().orc_splitmix64(ctypes.byref(st)) for _ in range(count)]

@@ -1,0 +1,190 @@
+"""GPU checks of the initial conditions, .raw snapshots and the command-line
+driver:
+
+  - fg2_init on the device against the oracle's restatement of the
+    reference's init.h functions: within one FP32 ulp (both evaluate the
+    same FP64 expression and round once; only the math libraries' last FP64
+    place can differ);
+  - fg2_dump / fg2_load: the file is the downloaded state under the
+    reference's header, load restores state and geometry, and a header of
+    another grid is FG2_ERR_SHAPE;
+  - fg2run: frames written as src/solve.cpp does, bit-identical to the same
+    loop through the C ABI; a run resumed from frame 1 writes a frame 2
+    bit-identical to the uninterrupted run's; frame 1 against the FP64
+    oracle to relative L2 1e-5; a NaN writes ****.raw and exits 255;
+    name=val sets the module parameter before the initial condition.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from fg2_amd import raw
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FG2RUN = os.path.join(ROOT, "fg2_amd", "fg2run")
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _ulp_close(gpu, orc):
+    o32 = np.abs(orc).astype(np.float32)
+    return np.abs(gpu.astype(np.float64) - orc) <= np.spacing(o32).astype(np.float64)
+
+
+@pytest.mark.parametrize("module,name,n1,n2,params", [
+    ("hydro", "blast", 200, 136, None),
+    ("hydro", "bow", 160, 160, None),
+    ("hydro", "implos", 96, 130, None),
+    ("hydro", "KH", 200, 136, None),
+    ("hydro", "Sod", 128, 100, None),
+    ("hydro", "zeros", 40, 50, None),
+    ("hydro-isot", "Kf", 128, 96, {"f": 0.02, "k": 4.0}),
+    ("hydro-isot", "default", 32, 32, None),
+    ("disk", "Hawley", 256, 192, None),
+])
+def test_init_matches_oracle(fg2, module, name, n1, n2, params):
+    from oracle.oracle import Oracle
+    _say(f"case init {module} {name} {n1}x{n2}")
+    seed = 1234
+    r = fg2.Run(module, n1, n2, params=params)
+    r.init(name, seed)
+    g = r.download()
+    o = Oracle(module, n1, n2, params=params)
+    o.init(name, seed)
+    ref = o.download()
+    ok = _ulp_close(g, ref)
+    assert ok.all(), (np.argwhere(~ok)[:5], g[~ok][:5], ref[~ok][:5])
+    # the device draws depend on the seed
+    if name in ("KH", "Kf"):
+        r.init(name, seed + 1)
+        assert not np.array_equal(r.download(), g)
+
+
+def test_dump_load_round_trip(fg2, tmp_path):
+    _say("case dump/load hydro 64x48")
+    r = fg2.Run("hydro", 64, 48, geometry=dict(l1=2.0, l2=1.5, p1=1, p2=0, c=0.4))
+    r.init("KH", 3)
+    x = r.download()
+    p = str(tmp_path / "0005.raw")
+    r.dump(p, 0.75)
+    s = raw.read(p)
+    assert np.array_equal(s.data, x)
+    assert (s.time, s.l1, s.l2, s.p1, s.p2) == (0.75, 2.0, 1.5, 1, 0)
+    assert np.float32(s.c) == np.float32(0.4)
+    # load into a run of the same grid but another geometry
+    q = fg2.Run("hydro", 64, 48, config=False)
+    assert q.load(p) == 0.75
+    q.config()
+    assert np.array_equal(q.download(), x)
+    geo = q.geometry()
+    assert (geo["l1"], geo["l2"], geo["p1"], geo["p2"]) == (2.0, 1.5, 1, 0)
+    # a snapshot of another grid is refused
+    w = fg2.Run("hydro", 48, 64)
+    with pytest.raises(fg2.Fg2Error) as e:
+        w.load(p)
+    assert e.value.code == fg2.ERR_SHAPE and "not compatible" in str(e.value)
+    w2 = fg2.Run("hydro-isot", 64, 48)
+    with pytest.raises(fg2.Fg2Error) as e:
+        w2.load(p)
+    assert e.value.code == fg2.ERR_SHAPE
+
+
+def _fg2run(args, cwd, timeout=120):
+    _say("fg2run " + " ".join(args))
+    r = subprocess.run([FG2RUN, *args], cwd=cwd, capture_output=True, text=True,
+                       timeout=timeout)
+    _say(r.stdout[-2000:])
+    return r
+
+
+def _solve_frames(fg2, module, n, name, seed, l0, frames):
+    """The frame loop of src/solve.cpp:32-58 through the Python mirror."""
+    r = fg2.Run(module, n, n)
+    r.init(name, seed)
+    t, out = 0.0, [r.download()]
+    dt_dump = l0 / frames
+    for i in range(1, frames + 1):
+        target = l0 - (frames - i) * dt_dump
+        while t < target:
+            dt = min(target - t, r.getdt())
+            assert r.step(t, dt) == 0
+            t = min(t + dt, target)
+        out.append(r.download())
+    return out
+
+
+def test_driver_frames_and_resume(fg2, tmp_path):
+    if not os.path.exists(FG2RUN):
+        pytest.fail("fg2run is not built")
+    a = tmp_path / "a"
+    b = tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    # as in the reference's parser, an option's value list runs until an
+    # argument that starts with '-' and a letter, so options go last
+    r = _fg2run(["hydro", "seed=9", "KH", "-n", "2", "64", "-l", "0.25"], a)
+    assert r.returncode == 0, r.stderr
+    assert "Done  Simulation" in r.stdout and "ms/step" in r.stdout
+    f = [raw.read(a / f"{i:04d}.raw") for i in range(3)]
+    assert [s.time for s in f] == [0.0, 0.125, 0.25]
+    # the same frame loop through the C ABI
+    ref = _solve_frames(fg2, "hydro", 64, "KH", 9, 0.25, 2)
+    for i in range(3):
+        np.testing.assert_array_equal(f[i].data, ref[i])
+    # resume from frame 1: same frame 2
+    shutil.copy(a / "0001.raw", b / "0001.raw")
+    r = _fg2run(["hydro", "0001.raw", "-n", "1", "64", "-l", "0.125"], b)
+    assert r.returncode == 0, r.stderr
+    assert "start at frame 1, time 0.125" in r.stdout
+    g = raw.read(b / "0002.raw")
+    assert g.time == 0.25
+    np.testing.assert_array_equal(g.data, f[2].data)
+    # frame 1 against the FP64 oracle over the same dt sequence
+    from oracle.oracle import Oracle
+    o = Oracle("hydro", 64, 64)
+    o.init("KH", 9)
+    t, dt0 = 0.0, o.getdt()
+    while t < 0.125:
+        dt = min(0.125 - t, dt0)
+        assert o.step(t, dt) == 0
+        t = min(t + dt, 0.125)
+    ref = o.download()
+    err = rel_l2(f[1].data, ref)
+    # u2 starts at zero (KH perturbs u1 only) and grows from the 1% noise, so
+    # its own norm is ~100x below the flow's: bound it against the velocity
+    flow = np.sqrt(np.sum(ref[..., 1:3] ** 2))
+    err_u2 = np.sqrt(np.sum((f[1].data[..., 2] - ref[..., 2]) ** 2)) / flow
+    assert err[0] < 1e-5 and err[1] < 1e-5 and err[3] < 1e-5 and err_u2 < 1e-6, (err, err_u2)
+
+
+def test_driver_parameters_and_crash_dump(fg2, tmp_path):
+    if not os.path.exists(FG2RUN):
+        pytest.fail("fg2run is not built")
+    r = _fg2run(["hydro", "gamma=1.4", "--nus=1e-4", "blast", "-n", "0", "32"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "Parameter    : gamma = 1.4" in r.stdout and "nus = 0.0001" in r.stdout
+    s = raw.read(tmp_path / "0000.raw")
+    p = (np.float32(1.4) - 1.0) * np.exp(s.data[..., 0] + s.data[..., 3].astype(np.float64))
+    assert np.allclose(np.sort(np.unique(np.round(p, 6))), [0.01, 1.0])
+    r = _fg2run(["hydro", "nosuch=1", "-n", "0", "32"], tmp_path)
+    assert 'Fail to set  : "nosuch=1"' in r.stdout
+    # a NaN in the input: the step fails, ****.raw holds the time of the failing step
+    x = np.zeros((32, 32, 4), np.float32)
+    x[5, 7, 1] = np.nan
+    raw.write(tmp_path / "0003.raw", raw.Snapshot(x, 0.5))
+    r = _fg2run(["hydro", "0003.raw", "-n", "1", "32", "-l", "0.01"], tmp_path)
+    assert r.returncode == 255
+    assert "crashed in 1 step" in r.stdout and "Abort Simulation" in r.stdout
+    assert "NaN" in r.stderr
+    c = raw.read(tmp_path / "****.raw")
+    assert c.time == 0.5 and c.shape == (32, 32, 4)

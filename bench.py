@@ -1,14 +1,18 @@
 """Benchmark of the fused RK3 time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--no-batch] [--no-serve] [--no-disk] [--disk-only]
 
-Workload: hydro module, Kelvin-Helmholtz shear layer on a 3584 x 3584
-periodic grid, FP32, fused kick+drift substages (three launches per RK3
-step), fixed dt from the CFL limit of the initial state.  Inputs are
-resident in HBM before the timed region.  Prints one JSON line with the
-whole-job cell-updates/s, the roofline of the fused substage kernel and the
-host CPU baseline (the oracle's FP32 instantiation on all host cores, on a
-bounded sample of the same workload).
+Workload (the `value`): hydro module, Kelvin-Helmholtz shear layer on a
+3584 x 3584 periodic grid, FP32, one launch of the one-pass step kernel per
+RK3 step (the three substages temporally blocked), fixed dt from the CFL
+limit of the initial state.  Inputs are resident in HBM before the timed
+region.  Prints one JSON line with the whole-job cell-updates/s, the
+roofline of the step kernel, and beside it: the substage path on the same
+workload, 64 batched hydro-isot runs, a served request stream, the disk
+module on 2048 x 2048 (Neumann / reflecting boundaries, three substage
+launches per step), and the host CPU baseline (the oracle's FP32
+instantiation on all host cores, on a bounded sample of the same workload).
 """
 import argparse
 import ctypes
@@ -219,6 +223,48 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=2, tile_
             "nan_requests": int(bad)}
 
 
+def bench_disk(ev, n=2048, steps=200, warmup=10):
+    """The disk module on its (log r, theta) grid, n x n, Neumann in r and
+    reflecting in theta, so three substage launches per RK3 step: the
+    Keplerian disc of the precision study (Newtonian point mass, es = 1e6),
+    fixed dt = 0.9 x the FP64 CFL step of the initial state."""
+    import fg2_amd
+    from fg2_amd import ics
+    x = ics.disk_kepler(n, n, seed=4, h_over_r=0.05)
+    run = fg2_amd.Run("disk", n, n, params={"es": 1.0e6, "rS": 0.0})
+    run.upload(x)
+    dt = 0.9 * run.getdt_cfl()
+    stream = run.stream()
+    for k in range(warmup):
+        run.step_async(k * dt, dt)
+    assert run.check() == 0
+    e0, e1 = ev.event(), ev.event()
+    ev.sync(stream)
+    t0 = time.perf_counter()
+    ev.record(e0, stream)
+    for k in range(steps):
+        run.step_async(k * dt, dt)
+    ev.record(e1, stream)
+    ev.sync(stream)
+    wall = time.perf_counter() - t0
+    gpu_ms = ev.elapsed_ms(e0, e1)
+    status = run.check()
+    tile_h, waves = run.tile()
+    run.close()
+    cells = n * n
+    alg = 10 * 6 * 4 * cells
+    launch_s = gpu_ms * 1e-3 / (3 * steps)
+    return {"workload": f"disk Keplerian disc {n}x{n}, Neumann r / reflecting theta, FP32, "
+                        "three substage launches per RK3 step, fixed 0.9 x CFL dt",
+            "steps": steps, "dt": dt, "status": status,
+            "ms_per_step": 1e3 * wall / steps,
+            "cell_updates_per_s": cells * steps / wall,
+            "hbm_frac_alg": cells * steps / wall * 240 / HBM_PEAK,
+            "kernel": "substage_kernel<Disk,stage,waves>",
+            "avg_launch_us": launch_s * 1e6,
+            "tile": {"tile_h": tile_h, "waves": waves}}
+
+
 def cpu_config1(steps=100):
     """Config 1 on the host: hydro 256^2 periodic, Gaussian random velocity
     (|k| <= 4, rms 0.01 c_s), closed-form dt, `steps` RK3 steps, with the
@@ -252,6 +298,9 @@ def main():
     ap.add_argument("--waves", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-disk", action="store_true")
+    ap.add_argument("--disk-only", action="store_true",
+                    help="run only the disk leg and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -262,6 +311,10 @@ def main():
     from fg2_amd import ics
 
     ev = HipEvents()
+    if args.disk_only:
+        print(json.dumps({"disk_2048": bench_disk(ev, steps=args.steps, warmup=args.warmup)}),
+              flush=True)
+        return
     x = ics.hydro_kh(N, N, seed=2)
     run = fg2_amd.Run("hydro", N, N, geometry=dict(l1=1.0, l2=1.0, p1=1, p2=1, c=0.5))
     run.set_tile(args.tile, args.waves)
@@ -340,6 +393,7 @@ def main():
         "substage_path_ms_per_step": substage_ms,
         "batched": None,
         "serving": None,
+        "disk_2048": None,
         "cpu_baseline": None,
     }
     run.close()
@@ -347,6 +401,8 @@ def main():
         out["batched"] = bench_batched(ev)
     if not args.no_serve:
         out["serving"] = bench_serving()
+    if not args.no_disk:
+        out["disk_2048"] = bench_disk(ev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_baseline"]["config1_hydro_256"] = cpu_config1()

@@ -118,6 +118,7 @@ struct fg2_run {
   DeviceDiag *diag = nullptr;
   DeviceDiag *host_diag = nullptr;               // pinned readback
   double *sum_partial = nullptr, *sum_out = nullptr;
+  float *rowtab = nullptr;       // disk: per-row table (Params::rowtab)
   hipStream_t stream = nullptr;
   int path = FG2_PATH_FUSED, dt_mode = FG2_DT_CLOSED;
   FusedShape shape{0, 1};
@@ -132,13 +133,45 @@ struct fg2_run {
 
 namespace {
 
-int auto_tile_h(int n1, int n2)
+// Resident workgroups per CU of a kernel family (kind 0: substage kernels,
+// kind 1: one-pass step kernel), from the hardware occupancy of the
+// compiled kernels; cached per (module, kind, waves).
+int blocks_per_cu(int module, int kind, int waves)
 {
-  // aim for about 3072 wavefront tiles (12 per CU on 256 CUs)
-  const long nstrips = (n2 + STRIP - 1) / STRIP;
-  long h = ((long)n1 * nstrips + 3071) / 3072;
-  h = std::max(16L, std::min(256L, h));
-  return (int)std::min<long>(h, n1);
+  static std::map<std::tuple<int, int, int>, int> cache;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_tuple(module, kind, waves);
+  auto it = cache.find(key);
+  if(it != cache.end()) return it->second;
+  int nb = 0;
+  if(kernel_occupancy(module, kind, waves, &nb) != hipSuccess || nb < 1) nb = 1;
+  cache[key] = nb;
+  return nb;
+}
+
+// Rows per tile that minimise (rounds of resident tiles) x (cost of a tile),
+// a tile costing its rows plus `extra` (the halo rows it loads or
+// recomputes).  Filling whole rounds matters: the kernels are VALU bound,
+// and a last round that is half full leaves half the SIMDs idle.
+int tile_rows(long n1, long nstrips, long slots, long h_min, long extra)
+{
+  long best_h = std::max(1L, std::min(n1, h_min)), best_cost = -1;
+  for(long h = h_min; h <= std::min(n1, 1024L); ++h) {
+    const long tiles = nstrips * ((n1 + h - 1) / h);
+    const long cost = ((tiles + slots - 1) / slots) * (h + extra);
+    if(best_cost < 0 || cost < best_cost) { best_cost = cost; best_h = h; }
+  }
+  return (int)best_h;
+}
+
+// Substage kernels: a tile is one wavefront's 58-column strip, loading 6
+// halo rows besides its own.
+int auto_tile_h(const fg2_run *r, int waves)
+{
+  const long nstrips = (r->n2 + STRIP - 1) / STRIP;
+  const long slots = (long)blocks_per_cu(r->module, 0, waves) * r->cus * waves;
+  return tile_rows(r->n1, nstrips, slots, 8, 3);
 }
 
 Params make_params(const fg2_run *r)
@@ -162,6 +195,7 @@ Params make_params(const fg2_run *r)
   P.blap0 = (float)(c2[0] / (D1 * D1) + c2[0] / (D2 * D2));
   for(size_t k = 0; k < r->par.size() && k < 16; ++k) P.par[k] = (float)r->par[k];
   P.rand1 = r->rand1; P.rand2 = r->rand2;
+  P.rowtab = r->rowtab;
   return P;
 }
 
@@ -192,8 +226,8 @@ int check_run(const fg2_run *r)
 FusedShape shape_of(const fg2_run *r)
 {
   FusedShape sh = r->shape;
-  if(r->tile_auto || sh.tile_h < 1) sh.tile_h = auto_tile_h(r->n1, r->n2);
   if(sh.waves != 1 && sh.waves != 2 && sh.waves != 4) sh.waves = 1;
+  if(r->tile_auto || sh.tile_h < 1) sh.tile_h = auto_tile_h(r, sh.waves);
   return sh;
 }
 
@@ -208,15 +242,13 @@ FusedShape step_shape_of(const fg2_run *r)
   FusedShape sh = r->shape;
   if(sh.waves != 1 && sh.waves != 2 && sh.waves != 4) sh.waves = 4;
   if(r->tile_auto || sh.tile_h < 1) {
+    // a workgroup tile of 64 NW - 18 output columns; stage 0 recomputes 12
+    // halo rows and stage 1 six, so a tile costs about its rows + 6
     sh.waves = 4;
-    // one resident round of workgroups: the kernel holds 2 workgroups of 4
-    // wavefronts per CU (its VGPR budget), so aim for 2 x CUs tiles, with at
-    // least 32 rows per tile to bound the 12 recomputed halo rows
     const long out = 64 * sh.waves - 6 * HALF;
     const long nstrips = (r->n2 + out - 1) / out;
-    const long chunks = std::max(1L, 2L * r->cus / nstrips);
-    sh.tile_h = (int)std::max(32L, (r->n1 + chunks - 1) / chunks);
-    sh.tile_h = std::min(sh.tile_h, r->n1);
+    const long slots = (long)blocks_per_cu(r->module, 1, sh.waves) * r->cus;
+    sh.tile_h = tile_rows(r->n1, nstrips, slots, 16, 6);
   }
   return sh;
 }
@@ -250,6 +282,16 @@ double bits_to_double(unsigned long long b)
   double d;
   memcpy(&d, &b, sizeof d);
   return d;
+}
+
+// Refill the disk row table after a change of Delta1, M, rS or rin; the
+// stream is drained so batches on other streams see the table.
+int refresh_rows(fg2_run *r)
+{
+  if(!r->rowtab || !r->configured) return FG2_OK;
+  HIP_TRY(launch_disk_rows(r->rowtab, make_params(r), r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return FG2_OK;
 }
 
 int substages(fg2_run *r, double t, double dt, int cfl)
@@ -324,6 +366,7 @@ int fg2_setup(fg2_run **out, const char *module, int n1, int n2, size_t *bytes)
   if(e == hipSuccess) e = hipHostMalloc((void **)&r->host_diag, sizeof(DeviceDiag), hipHostMallocDefault);
   if(e == hipSuccess) e = hipMalloc((void **)&r->sum_partial, 4 * (size_t)n1 * sizeof(double));
   if(e == hipSuccess) e = hipMalloc((void **)&r->sum_out, 4 * sizeof(double));
+  if(e == hipSuccess && m == DISK) e = hipMalloc((void **)&r->rowtab, 4 * (size_t)n1 * sizeof(float));
   int dev = 0;
   if(e == hipSuccess) e = hipGetDevice(&dev);
   if(e == hipSuccess) e = hipDeviceGetAttribute(&r->cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -348,6 +391,7 @@ void fg2_free(fg2_run *r)
   if(r->host_diag) (void)hipHostFree(r->host_diag);
   if(r->sum_partial) (void)hipFree(r->sum_partial);
   if(r->sum_out) (void)hipFree(r->sum_out);
+  if(r->rowtab) (void)hipFree(r->rowtab);
   if(r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
 }
@@ -363,7 +407,7 @@ int fg2_set_geometry(fg2_run *r, double l1, double l2, int p1, int p2, double c)
     r->Delta1 = (float)r->l1 / (float)r->n1;
     r->Delta2 = (float)r->l2 / (float)r->n2;
   }
-  return FG2_OK;
+  return refresh_rows(r);
 }
 
 int fg2_get_geometry(const fg2_run *r, double *out)
@@ -390,7 +434,7 @@ int fg2_config(fg2_run *r)
   r->Delta1 = (float)r->l1 / (float)r->n1;
   r->Delta2 = (float)r->l2 / (float)r->n2;
   r->configured = true;
-  return FG2_OK;
+  return refresh_rows(r);
 }
 
 const char *fg2_para(fg2_run *r, const char *in)
@@ -405,6 +449,7 @@ const char *fg2_para(fg2_run *r, const char *in)
     if(name == defs[k].name) {
       const float val = (float)atof(eq + 1);     // stored as R (src/para.cpp:33)
       r->par[k] = val;
+      if(refresh_rows(r)) return nullptr;
       snprintf(r->para_buf, sizeof r->para_buf, "%s = %g", name.c_str(), (double)val);
       return r->para_buf;
     }

@@ -60,6 +60,9 @@ struct Params {
   float par[16];
   // hydro-isot first_order random phases (src/hydro-isot/Euler.cuh:24-25)
   float rand1, rand2;
+  // disk: per-row table of the run, 4 floats per row i:
+  // {sphr = rin e^{(i+1/2) Delta1}, 1/sphr, sphr^2, M / (sphr - rS)^2}
+  const float *rowtab;
 };
 
 // Diagnostics written by the kernels.  flags bit 0: a NaN was produced

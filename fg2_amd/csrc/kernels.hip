@@ -707,6 +707,27 @@ hipError_t launch_drift(int module, float *u, const float *v, long s, const Para
   });
 }
 
+// Per-row table of the disk grid (Params::rowtab): the row's spherical
+// radius (src/disk/eqns.cuh:52-54) and the gravity of the point mass
+// (src/disk/eqns.cuh:238-239), in the reference's expressions.
+__global__ __launch_bounds__(256)
+void disk_rows_kernel(float *tab, const int n1, const float Delta1, const float M, const float rS,
+                      const float rin)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if(i >= n1) return;
+  const float sphr = rin * expf(((float)i + 0.5f) * Delta1);
+  const float tmp = sphr - rS;
+  reinterpret_cast<float4 *>(tab)[i] = make_float4(sphr, 1.0f / sphr, sphr * sphr, M / (tmp * tmp));
+}
+
+hipError_t launch_disk_rows(float *tab, const Params &P, hipStream_t st)
+{
+  disk_rows_kernel<<<(P.n1 + 255) / 256, 256, 0, st>>>(tab, P.n1, P.Delta1, P.par[0], P.par[1],
+                                                       P.par[2]);
+  return hipGetLastError();
+}
+
 hipError_t launch_euler(int module, float *u, long s, const Params &P, float dt, hipStream_t st)
 {
   return by_module(module, [&](auto m) {
@@ -794,6 +815,45 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
     }
     return hipGetLastError();
     }
+  });
+}
+
+// Resident workgroups per CU of a kernel family (hardware occupancy from its
+// register and LDS use): kind 0 = the substage kernels (the least of the
+// three stages), kind 1 = the one-pass step kernel.
+hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
+{
+  *blocks_per_cu = 0;
+  if(waves != 1 && waves != 2 && waves != 4) return hipErrorInvalidValue;
+  return by_module(module, [&](auto m) -> hipError_t {
+    using M = decltype(m);
+    int nb = 0;
+    hipError_t e = hipSuccess;
+    auto take = [&](const void *fn) {
+      int b = 0;
+      if(e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, 64 * waves, 0);
+      nb = nb ? std::min(nb, b) : b;
+    };
+    if(kind == 0) {
+#define FG2_W(WPB)                                                                 \
+      if(waves == WPB) {                                                           \
+        take((const void *)substage_kernel<M, 0, WPB>);                            \
+        take((const void *)substage_kernel<M, 1, WPB>);                            \
+        take((const void *)substage_kernel<M, 2, WPB>);                            \
+      }
+      FG2_W(1) FG2_W(2) FG2_W(4)
+#undef FG2_W
+    } else {
+      if constexpr(M::ID == DISK) {
+        return hipErrorInvalidValue;
+      } else {
+        if(waves == 1) take((const void *)step_kernel<M, 1>);
+        if(waves == 2) take((const void *)step_kernel<M, 2>);
+        if(waves == 4) take((const void *)step_kernel<M, 4>);
+      }
+    }
+    *blocks_per_cu = nb;
+    return e;
   });
 }
 

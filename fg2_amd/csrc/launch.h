@@ -37,6 +37,15 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
                        double dt, const FusedShape &shape, int cfl, DeviceDiag *diag,
                        hipStream_t st);
 
+// Fill the disk row table (Params::rowtab, 4 floats per row) of a run
+// from its Delta1 and parameters M, rS, rin.
+hipError_t launch_disk_rows(float *tab, const Params &P, hipStream_t st);
+
+// Resident workgroups per CU (hipOccupancyMaxActiveBlocksPerMultiprocessor)
+// of the substage kernels (kind 0, the least over the three stages) or of
+// the one-pass step kernel (kind 1) with `waves` wavefronts per workgroup.
+hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu);
+
 // CFL rates (FP64 max over cells of the advective and diffusive rates)
 // of a state, folded into diag->lam_*_bits.
 hipError_t launch_cfl(int module, const float *u, long s, const Params &P, DeviceDiag *diag,

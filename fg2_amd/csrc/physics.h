@@ -207,25 +207,34 @@ struct Disk {
   static constexpr int D12V[ND] = {0, 1, 2, 5};
   using P_t = Prim<NV, ND>;
 
+  // Row quantities of the (log r, theta) grid, from the run's table (one
+  // scalar load: the row index is uniform across a wavefront).
+  static __device__ __forceinline__ float4 row(const Params &P, int i)
+  { return reinterpret_cast<const float4 *>(P.rowtab)[i]; }
+
   static __device__ __forceinline__ void rhs(const Params &P, const P_t &x, int i, int j, float dt[NV])
   {
     enum { LND, UR, UZ, OMG, LNE, LNER };
-    const float M = P.par[0], rS = P.par[1], rin = P.par[2], g = P.par[3];
+    const float M = P.par[0], g = P.par[3];
     const float nus = P.par[4], nub = P.par[5], kap = P.par[6], alpha = P.par[7];
     const float ar = P.par[8], es = P.par[9], ff = P.par[10];
     const float pdd = P.par[11], pnu = P.par[12], ped = P.par[13], prd = P.par[14];
 
+    // sin and cos of the column and 1/sin depend on j only (hoisted out of a
+    // row march); sphr, 1/sphr, sphr^2 and the gravity come from the table.
+    // Reciprocals are v_rcp_f32 (1 ulp) instead of IEEE divisions.
     float sint, cost;
     sincosf(((float)j + 0.5f) * P.Delta2, &sint, &cost);
-    const float sphr = rin * expf(((float)i + 0.5f) * P.Delta1);
+    const float4 rw = row(P, i);
+    const float sphr = rw.x, isphr = rw.y, sphr_2 = rw.z, gr = rw.w;
     const float r = sphr * sint;
-    const float sphr_2 = sphr * sphr;
-    const float isphr = 1.0f / sphr, ir = 1.0f / r;
+    const float ir = isphr * __builtin_amdgcn_rcpf(sint);
 
     const float den = expf(x.c[LND]), eng = expf(x.c[LNE]), rad = expf(x.c[LNER]);
+    const float ieng = __builtin_amdgcn_rcpf(eng);
     const float gamma1 = g - 1.0f;
     const float temp = gamma1 * eng;
-    const float nuSS = alpha * g * temp * r * sqrtf(r / M);
+    const float nuSS = alpha * g * temp * r * __builtin_amdgcn_sqrtf(r * __builtin_amdgcn_rcpf(M));
 
     float dr[NV], dz[NV], lap[NV], d11[NV], d22[NV];
 #pragma unroll
@@ -283,7 +292,7 @@ struct Disk {
       dt[OMG] += two_nus * (Spr * dr[LND] + Szp * dz[LND]) * ir;
       dt[LNE] += (two_nus * (Srr * Srr + Szz * Szz + Spp * Spp +
                   2.0f * (Srz * Srz + Szp * Szp + Spr * Spr)) +
-                 nub * div_u * div_u) / eng;
+                 nub * div_u * div_u) * ieng;
     }
     {                                                                     // velocity diffusion
       const float tmp1 = (nus + nuSS) + pnu * sphr_2;
@@ -301,25 +310,31 @@ struct Disk {
       dt[LND] += lap_d_d * pdd * sphr_2;
       dt[LNE] += lap_e_e * (ped * sphr_2 + kap * gamma1 + g * nuSS);
 
-      const float kappa = ff * den * den * powf(temp, -3.5f);
+      // free-free opacity and cooling vanish with ff = 0 (the default): the
+      // uniform branch skips powf
+      float kappa = 0.0f, cooling = 0.0f;
+      if(ff != 0.0f) {
+        kappa = ff * den * den * powf(temp, -3.5f);
+        const float temp2 = temp * temp;
+        cooling = kappa * (ar * temp2 * temp2 * __builtin_amdgcn_rcpf(den) - rad);
+      }
       const float chi = es * den + kappa;
-      const float ichi = 1.0f / chi;
+      const float ichi = __builtin_amdgcn_rcpf(chi);
       const float d_lner_2 = dr[LNER] * dr[LNER] + dz[LNER] * dz[LNER];
-      const float bigR = sqrtf(d_lner_2) * ichi;
+      const float bigR = __builtin_amdgcn_sqrtf(d_lner_2) * ichi;
       const float denom = 6.0f + (3.0f + bigR) * bigR;
-      const float Lambda = (2.0f + bigR) / denom;
+      const float idenom = __builtin_amdgcn_rcpf(denom);
+      const float Lambda = (2.0f + bigR) * idenom;
       dt[UR] -= Lambda * rad * (dr[LND] + dr[LNE]);
       dt[UZ] -= Lambda * rad * (dz[LND] + dz[LNE]);
 
-      const float temp2 = temp * temp;
-      const float cooling = kappa * (ar * temp2 * temp2 / den - rad);
-      dt[LNE]  -= cooling / eng;
-      dt[LNER] += cooling / rad;
+      dt[LNE]  -= cooling * ieng;
+      dt[LNER] += cooling * __builtin_amdgcn_rcpf(rad);
 
       const float d_lnd_d_lner = dr[LND] * dr[LNER] + dz[LND] * dz[LNER];
       const float lap_er_er = lap[LNER] + d_lner_2;
       const float nu_rad = Lambda * ichi;
-      const float Lambdap = -bigR * (4.0f + bigR) / (denom * denom);
+      const float Lambdap = -bigR * (4.0f + bigR) * (idenom * idenom);
       const float dr_lnEr = dr[LND] + dr[LNER];
       const float dz_lnEr = dz[LND] + dz[LNER];
       const float drr_lnEr = drr_lnd + drr_lner;
@@ -327,7 +342,7 @@ struct Disk {
       const float dzz_lnEr = dzz_lnd + dzz_lner;
       const float dr_chi = chi * dr[LND] - 3.5f * kappa * dr[LNE];
       const float dz_chi = chi * dz[LND] - 3.5f * kappa * dz[LNE];
-      const float ibr = 1.0f / (1e-16f + bigR);
+      const float ibr = __builtin_amdgcn_rcpf(1e-16f + bigR);
       const float dr_bigR = ((dr_lnEr * drr_lnEr + dz_lnEr * drz_lnEr) * ibr * ichi - bigR * dr_chi) * ichi;
       const float dz_bigR = ((dr_lnEr * drz_lnEr + dz_lnEr * dzz_lnEr) * ibr * ichi - bigR * dz_chi) * ichi;
       dt[LNER] += lap_er_er * prd * sphr_2 +
@@ -335,12 +350,8 @@ struct Disk {
         dr_lnEr * (Lambdap * dr_bigR - nu_rad * dr_chi) * ichi +
         dz_lnEr * (Lambdap * dz_bigR - nu_rad * dz_chi) * ichi;
     }
-    {                                                                     // gravity
-      const float tmp = sphr - rS;
-      const float gr = M / (tmp * tmp);
-      dt[UR] -= sint * gr;
-      dt[UZ] -= cost * gr;
-    }
+    dt[UR] -= sint * gr;                                                  // gravity
+    dt[UZ] -= cost * gr;
   }
 
   // Radiation-equilibrium pre-step (src/disk/Euler.cuh:19-69).
@@ -374,19 +385,18 @@ struct Disk {
                                                   float k1a, float k1b)
   {
     const float g = P.par[3];
-    const float sphr = P.par[2] * expf(((float)i + 0.5f) * P.Delta1);
     const float temp = (g - 1.0f) * expf(c[4]);
     const float cs = sqrtf(g * temp + expf(c[5]) * THIRD);
-    return (fabsf(c[1]) + fabsf(c[2]) + cs) * (k1a + k1b) / sphr;
+    return (fabsf(c[1]) + fabsf(c[2]) + cs) * (k1a + k1b) * row(P, i).y;
   }
   // Largest local diffusivity in index-space units (coefficient / sphr^2).
   static constexpr bool CELL_DIFF = true;
   static __device__ __forceinline__ float lam_diff(const Params &P, const float *c, int i, int j)
   {
-    const float M = P.par[0], rin = P.par[2], g = P.par[3];
-    const float sphr = rin * expf(((float)i + 0.5f) * P.Delta1);
-    const float r = sphr * sinf(((float)j + 0.5f) * P.Delta2);
-    const float s2 = sphr * sphr;
+    const float M = P.par[0], g = P.par[3];
+    const float4 rw = row(P, i);
+    const float r = rw.x * sinf(((float)j + 0.5f) * P.Delta2);
+    const float s2 = rw.z;
     const float temp = (g - 1.0f) * expf(c[4]);
     const float nuSS = P.par[7] * g * temp * r * sqrtf(r / M);
     const float nuv = P.par[4] + nuSS;

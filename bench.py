@@ -1,7 +1,7 @@
 """Benchmark of the fused RK3 time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--no-batch] [--no-serve] [--no-disk] [--disk-only]
+                    [--no-batch] [--no-serve] [--no-disk] [--no-spectrum] [--disk-only]
 
 Workload (the `value`): hydro module, Kelvin-Helmholtz shear layer on a
 3584 x 3584 periodic grid, FP32, one launch of the one-pass step kernel per
@@ -11,7 +11,8 @@ region.  Prints one JSON line with the whole-job cell-updates/s, the
 roofline of the step kernel, and beside it: the substage path on the same
 workload, 64 batched hydro-isot runs, a served request stream, the disk
 module on 2048 x 2048 (Neumann / reflecting boundaries, three substage
-launches per step), and the host CPU baseline (the oracle's FP32
+launches per step), the energy and vorticity spectra of the 3584 x 3584
+state, and the host CPU baseline (the oracle's FP32
 instantiation on all host cores, on a bounded sample of the same workload).
 """
 import argparse
@@ -265,6 +266,24 @@ def bench_disk(ev, n=2048, steps=200, warmup=10):
             "tile": {"tile_h": tile_h, "waves": waves}}
 
 
+def bench_spectrum(n=N, calls=10):
+    """Energy and vorticity shell spectra (fg2_spectrum) of the hydro KH
+    state on n x n: pack u1 + i u2, one FP32 complex 2-D FFT, FP64 shell sums;
+    results copied to the host.  The first call (FFT plan) is untimed."""
+    import fg2_amd
+    run = fg2_amd.Run("hydro", n, n)
+    run.init("KH", 2)
+    s = run.spectrum()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        s = run.spectrum()
+    wall = (time.perf_counter() - t0) / calls
+    run.close()
+    return {"workload": f"energy + vorticity shell spectra of hydro KH {n}x{n} (device IC), "
+                        "FP32 FFT, FP64 shell sums, host copy of the spectra",
+            "ms_per_call": 1e3 * wall, "cells_per_s": n * n / wall, "shells": len(s["E"])}
+
+
 def cpu_config1(steps=100):
     """Config 1 on the host: hydro 256^2 periodic, Gaussian random velocity
     (|k| <= 4, rms 0.01 c_s), closed-form dt, `steps` RK3 steps, with the
@@ -299,6 +318,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-disk", action="store_true")
+    ap.add_argument("--no-spectrum", action="store_true")
     ap.add_argument("--disk-only", action="store_true",
                     help="run only the disk leg and print its JSON (profiling)")
     args = ap.parse_args()
@@ -394,6 +414,7 @@ def main():
         "batched": None,
         "serving": None,
         "disk_2048": None,
+        "spectrum": None,
         "cpu_baseline": None,
     }
     run.close()
@@ -403,6 +424,8 @@ def main():
         out["serving"] = bench_serving()
     if not args.no_disk:
         out["disk_2048"] = bench_disk(ev)
+    if not args.no_spectrum:
+        out["spectrum"] = bench_spectrum()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_baseline"]["config1_hydro_256"] = cpu_config1()

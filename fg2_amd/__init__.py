@@ -97,6 +97,8 @@ def _load():
         "fg2_sums": (I, [P, DP]),
         "fg2_perf_model": (I, [P, DP, DP, DP]),
         "fg2_init": (I, [P, ctypes.c_char_p, ctypes.c_ulonglong]),
+        "fg2_spectrum": (I, [P, DP, DP, ctypes.POINTER(ctypes.c_ulonglong), IP]),
+        "fg2_vorticity": (I, [P, FP]),
         "fg2_dump": (I, [P, ctypes.c_char_p, D]),
         "fg2_load": (I, [P, ctypes.c_char_p, DP]),
         "fg2_batch_create": (I, [PP, PP, I, I, I]),
@@ -282,6 +284,26 @@ class Run:
         t = ctypes.c_double(0.0)
         _check(lib.fg2_load(self.h, os.fsencode(path), ctypes.byref(t)))
         return t.value
+
+    # -- spectral diagnostics (tools/spec.pro, tools/vis.pro) ---------------
+    def spectrum(self):
+        """Shell-integrated energy and vorticity spectra of the velocity:
+        dict with k (shell centres 1..n), E, Z and counts per shell."""
+        n = ctypes.c_int(0)
+        _check(lib.fg2_spectrum(self.h, None, None, None, ctypes.byref(n)))
+        E = np.zeros(n.value)
+        Z = np.zeros(n.value)
+        c = np.zeros(n.value, np.uint64)
+        _check(lib.fg2_spectrum(self.h, _dp(E), _dp(Z),
+                                c.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong)),
+                                ctypes.byref(n)))
+        return {"k": np.arange(1, n.value + 1, dtype=np.float64), "E": E, "Z": Z, "counts": c}
+
+    def vorticity(self):
+        """Spectral vorticity field (n1, n2) of the velocity (tools/vis.pro)."""
+        out = np.empty((self.n1, self.n2), np.float32)
+        _check(lib.fg2_vorticity(self.h, _fp(out)))
+        return out
 
     # -- device operations (src/fg2.h:81-84) ------------------------------
     def euler(self, dt):

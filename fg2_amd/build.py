@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfg2_amd.so")
 COPY_EXE = os.path.join(HERE, "hbm_copy")
 DRIVER_EXE = os.path.join(HERE, "fg2run")
-SOURCES = ["kernels.hip", "capi.hip", "ics.hip"]
+SOURCES = ["kernels.hip", "capi.hip", "ics.hip", "spectra.hip"]
 HEADERS = ["fg2_layout.h", "physics.h", "launch.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -49,7 +49,7 @@ def build(verbose=False, jobs=2):
     while procs:
         _wait(procs.pop(0))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *_objects(), "-o", tmp]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *_objects(), "-o", tmp, "-lhipfft"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -119,6 +119,7 @@ struct fg2_run {
   DeviceDiag *host_diag = nullptr;               // pinned readback
   double *sum_partial = nullptr, *sum_out = nullptr;
   float *rowtab = nullptr;       // disk: per-row table (Params::rowtab)
+  SpectralWork *spec = nullptr;  // spectral diagnostics workspace, made on first use
   hipStream_t stream = nullptr;
   int path = FG2_PATH_FUSED, dt_mode = FG2_DT_CLOSED;
   FusedShape shape{0, 1};
@@ -392,6 +393,7 @@ void fg2_free(fg2_run *r)
   if(r->sum_partial) (void)hipFree(r->sum_partial);
   if(r->sum_out) (void)hipFree(r->sum_out);
   if(r->rowtab) (void)hipFree(r->rowtab);
+  if(r->spec) spectral_free(r->spec);
   if(r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
 }
@@ -829,6 +831,46 @@ int fg2_sums(fg2_run *r, double *out)
   HIP_TRY(hipMemcpyAsync(out, r->sum_out, 4 * sizeof(double), hipMemcpyDeviceToHost, r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
   return FG2_OK;
+}
+
+// Spectral diagnostics of tools/spec.pro, tools/vis.pro (see spectra.hip).
+static int spectral_status(int code)
+{
+  switch(code) {
+  case SPECTRA_OK: return FG2_OK;
+  case SPECTRA_FFT: return fail(FG2_ERR_HIP, "hipFFT plan or execution failed");
+  case SPECTRA_NOMEM: return fail(FG2_ERR_NOMEM, "spectral workspace allocation failed");
+  case SPECTRA_SHAPE: return fail(FG2_ERR_SHAPE, "too many spectral shells for one workgroup's LDS");
+  default: return fail(FG2_ERR_HIP, "spectral kernels: %s", hipGetErrorString(hipGetLastError()));
+  }
+}
+
+static int spectral_work(fg2_run *r)
+{
+  if(r->spec) return FG2_OK;
+  return spectral_status(spectral_create(&r->spec, r->n1, r->n2));
+}
+
+int fg2_spectrum(fg2_run *r, double *E, double *Z, unsigned long long *counts, int *nbins)
+{
+  if(!r || !nbins) return fail(FG2_ERR_ARG, "NULL argument");
+  const int need = spectral_bins(r->n1, r->n2);
+  if((E || Z || counts) && *nbins < need) {
+    *nbins = need;
+    return fail(FG2_ERR_ARG, "spectrum has %d shells, arrays hold fewer", need);
+  }
+  *nbins = need;
+  if(!E && !Z && !counts) return FG2_OK;
+  if(int e = spectral_work(r)) return e;
+  return spectral_status(spectral_shells(r->spec, r->in[r->cur], r->s, r->nv, E, Z, counts,
+                                         r->stream));
+}
+
+int fg2_vorticity(fg2_run *r, float *host)
+{
+  if(!r || !host) return fail(FG2_ERR_ARG, "NULL argument");
+  if(int e = spectral_work(r)) return e;
+  return spectral_status(spectral_vorticity(r->spec, r->in[r->cur], r->s, r->nv, host, r->stream));
 }
 
 int fg2_perf_model(const fg2_run *r, double *flops, double *bits, double *alg)

@@ -63,6 +63,23 @@ enum IcId { IC_ZEROS = 0, IC_BLAST, IC_BOW, IC_IMPLOS, IC_KH, IC_SOD, IC_KF, IC_
 hipError_t launch_ic(int module, int ic, float *u, long s, int n1, int n2, float l1, float l2,
                      const float *par, unsigned long long seed, hipStream_t st);
 
+// Spectral diagnostics (spectra.hip).  Status codes of the spectral_*
+// functions:
+enum { SPECTRA_OK = 0, SPECTRA_HIP = 1, SPECTRA_FFT = 2, SPECTRA_NOMEM = 3, SPECTRA_SHAPE = 4 };
+struct SpectralWork;
+int spectral_bins(int n1, int n2);               // shells below the Galerkin cutoff
+int spectral_create(SpectralWork **out, int n1, int n2);
+void spectral_free(SpectralWork *W);
+// Shell-summed energy E and vorticity Z spectra and the wavevectors per
+// shell of the velocity (variables 1, 2) of an AoS state; host outputs of
+// spectral_bins() entries each (any may be NULL); synchronises the stream.
+int spectral_shells(SpectralWork *W, const float *u, long s, int nv, double *E, double *Z,
+                    unsigned long long *counts, hipStream_t st);
+// Spectral vorticity field, n1 x n2 floats row-major, copied to `host`
+// (may be NULL); synchronises the stream.
+int spectral_vorticity(SpectralWork *W, const float *u, long s, int nv, float *host,
+                       hipStream_t st);
+
 // Batched path: many independent runs in one persistent launch.
 struct BatchRun {
   const float *uin; float *uout; float *v;

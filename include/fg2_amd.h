@@ -102,6 +102,25 @@ int fg2_download(fg2_run *run, float *host);
 int fg2_upload_v(fg2_run *run, const float *host);
 int fg2_download_v(fg2_run *run, float *host);
 
+/* ---- spectral diagnostics (tools/spec.pro, tools/vis.pro) ---- */
+
+/* Shell-integrated spectra of the state's velocity (variables 1 and 2) on
+ * integer wavenumbers (tools/getk.pro:18-33): E[b] = sum over the shell of
+ * (|u1^|^2 + |u2^|^2) / 2 (tools/spec.pro:25-31), Z[b] = sum of
+ * |omega^|^2 / 2 with omega^ = i k1 u2^ - i k2 u1^ and the Nyquist
+ * wavenumbers zeroed (tools/spec.pro:58-63), counts[b] = wavevectors in the
+ * shell; u^ is the forward DFT divided by n1 n2, as IDL's fft.  Shell b
+ * holds |k| in [b + 1/2, b + 3/2), shell 0 holds (0, 3/2), the last shell is
+ * closed, b < n = ceil(min(n1, n2) / 2 + 0.99) (tools/oned.pro:18-44).
+ * On input *nbins is the length of the arrays (each may be NULL); on
+ * output it is n.  Too short arrays are FG2_ERR_ARG.  FP32 FFT (hipFFT),
+ * FP64 shell sums.  Replaces the IDL post-processing of .raw frames. */
+int fg2_spectrum(fg2_run *run, double *E, double *Z, unsigned long long *counts, int *nbins);
+
+/* Spectral vorticity field of the state, real part of the inverse DFT of
+ * omega^ (tools/vis.pro:57-63), n1*n2 floats row-major into `host`. */
+int fg2_vorticity(fg2_run *run, float *host);
+
 /* ---- initial conditions and .raw snapshots ---- */
 
 /* Fill the state with the module's initial condition `name`, evaluated on

@@ -166,6 +166,13 @@ int tile_rows(long n1, long nstrips, long slots, long h_min, long extra)
   return (int)best_h;
 }
 
+// Persistent grid of a batched launch: one resident round of workgroups of
+// the batch kernel that runs (its hardware occupancy times the CUs).
+int batch_grid(int module, bool step, int waves, int cus)
+{
+  return std::max(1, blocks_per_cu(module, step ? 3 : 2, waves) * cus);
+}
+
 // Substage kernels: a tile is one wavefront's 58-column strip, loading 6
 // halo rows besides its own.
 int auto_tile_h(const fg2_run *r, int waves)
@@ -1025,10 +1032,8 @@ int fg2_batch_create(fg2_batch **out, fg2_run *const *runs, int nruns, int waves
   if(e == hipSuccess) e = hipGetDevice(&dev);
   if(e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if(e != hipSuccess) { fg2_batch_free(b); return fail(FG2_ERR_HIP, "batch setup: %s", hipGetErrorString(e)); }
-  // persistent grid: one resident round of workgroups (the step kernel's
-  // VGPR budget allows 8 wavefronts per CU, the substage kernel's 16)
   b->cus = cus;
-  b->grid = b->step ? cus * 8 / waves : cus * 16 / waves;
+  b->grid = batch_grid(b->module, b->step, waves, cus);
   *out = b;
   return FG2_OK;
 }
@@ -1055,7 +1060,7 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
   for(fg2_run *r : b->runs) HIP_TRY(hipStreamSynchronize(r->stream));
   std::vector<FusedShape> cand;
   const int hs[] = {16, 32, 48, 64, 96, 128};
-  const int ws_step[] = {4, 2}, ws_sub[] = {4, 2, 1};
+  const int ws_step[] = {4, 2, 1}, ws_sub[] = {4, 2, 1};
   if(b->step) { for(int w : ws_step) for(int h : hs) cand.push_back({h, w}); }
   else        { for(int w : ws_sub)  for(int h : hs) cand.push_back({h, w}); }
   hipEvent_t e0, e1;
@@ -1065,7 +1070,7 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
   FusedShape best{0, 0};
   for(const FusedShape &c : cand) {
     b->waves = c.waves;
-    b->grid = b->step ? b->cus * 8 / c.waves : b->cus * 16 / c.waves;
+    b->grid = batch_grid(b->module, b->step, c.waves, b->cus);
     for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = std::min(c.tile_h, b->runs[i]->n1);
     HIP_TRY(hipStreamSynchronize(b->stream));
     if(int e = batch_fill(b, dts)) return e;
@@ -1081,7 +1086,7 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->waves = best.waves;
-  b->grid = b->step ? b->cus * 8 / best.waves : b->cus * 16 / best.waves;
+  b->grid = batch_grid(b->module, b->step, best.waves, b->cus);
   for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = std::min(best.tile_h, b->runs[i]->n1);
   for(fg2_run *r : b->runs) reset_diag_kernel<<<1, 1, 0, b->stream>>>(r->diag, 0);
   HIP_TRY(hipStreamSynchronize(b->stream));
@@ -1359,7 +1364,7 @@ int server_launch(fg2_server *S)
                            hipMemcpyHostToDevice, S->stream));
     zero_ints_kernel<<<1, 64, 0, S->stream>>>(S->counters + slot, 1);
     HIP_TRY(launch_batch_step(m, S->rec_dev[slot], (int)S->rec_host.size(), begin, S->waves,
-                              S->cus * 8 / S->waves, S->counters + slot, S->stream));
+                              batch_grid(m, true, S->waves, S->cus), S->counters + slot, S->stream));
     HIP_TRY(hipEventRecord(S->rec_done[slot], S->stream));
   }
   for(Request *q : S->active) {

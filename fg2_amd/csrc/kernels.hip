@@ -820,7 +820,8 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
 
 // Resident workgroups per CU of a kernel family (hardware occupancy from its
 // register and LDS use): kind 0 = the substage kernels (the least of the
-// three stages), kind 1 = the one-pass step kernel.
+// three stages), kind 1 = the one-pass step kernel, kind 2 = the batched
+// substage kernels, kind 3 = the batched step kernel.
 hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
 {
   *blocks_per_cu = 0;
@@ -834,12 +835,17 @@ hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
       if(e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, 64 * waves, 0);
       nb = nb ? std::min(nb, b) : b;
     };
-    if(kind == 0) {
+    if(kind == 0 || kind == 2) {
 #define FG2_W(WPB)                                                                 \
-      if(waves == WPB) {                                                           \
+      if(waves == WPB && kind == 0) {                                              \
         take((const void *)substage_kernel<M, 0, WPB>);                            \
         take((const void *)substage_kernel<M, 1, WPB>);                            \
         take((const void *)substage_kernel<M, 2, WPB>);                            \
+      }                                                                            \
+      if(waves == WPB && kind == 2) {                                              \
+        take((const void *)batch_kernel<M, 0, WPB>);                               \
+        take((const void *)batch_kernel<M, 1, WPB>);                               \
+        take((const void *)batch_kernel<M, 2, WPB>);                               \
       }
       FG2_W(1) FG2_W(2) FG2_W(4)
 #undef FG2_W
@@ -847,9 +853,15 @@ hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
       if constexpr(M::ID == DISK) {
         return hipErrorInvalidValue;
       } else {
-        if(waves == 1) take((const void *)step_kernel<M, 1>);
-        if(waves == 2) take((const void *)step_kernel<M, 2>);
-        if(waves == 4) take((const void *)step_kernel<M, 4>);
+        if(kind == 1) {
+          if(waves == 1) take((const void *)step_kernel<M, 1>);
+          if(waves == 2) take((const void *)step_kernel<M, 2>);
+          if(waves == 4) take((const void *)step_kernel<M, 4>);
+        } else {
+          if(waves == 1) take((const void *)batch_step_kernel<M, 1>);
+          if(waves == 2) take((const void *)batch_step_kernel<M, 2>);
+          if(waves == 4) take((const void *)batch_step_kernel<M, 4>);
+        }
       }
     }
     *blocks_per_cu = nb;

@@ -42,8 +42,9 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
 hipError_t launch_disk_rows(float *tab, const Params &P, hipStream_t st);
 
 // Resident workgroups per CU (hipOccupancyMaxActiveBlocksPerMultiprocessor)
-// of the substage kernels (kind 0, the least over the three stages) or of
-// the one-pass step kernel (kind 1) with `waves` wavefronts per workgroup.
+// with `waves` wavefronts per workgroup of the substage kernels (kind 0, the
+// least over the three stages), the one-pass step kernel (kind 1), the
+// batched substage kernels (kind 2) or the batched step kernel (kind 3).
 hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu);
 
 // CFL rates (FP64 max over cells of the advective and diffusive rates)

@@ -123,23 +123,34 @@ void shell_kernel(const float2 *__restrict__ zh, const int n1, const int n2, con
   }
 }
 
+// 16 shells per workgroup, 16 slices of the workgroup rows per shell: slice
+// t adds rows t, t + 16, ... in order, then slice 0 adds the 16 slice sums in
+// order, so the sum order is fixed.
 __global__ __launch_bounds__(256)
 void shell_reduce_kernel(const double *__restrict__ partial, const unsigned int *__restrict__ pcount,
                          const int groups, const int nbins, double *__restrict__ sums,
                          unsigned long long *__restrict__ counts)
 {
-  const int b = blockIdx.x * 256 + threadIdx.x;
-  if(b >= nbins) return;
+  __shared__ double lE[16][16], lZ[16][16];
+  __shared__ unsigned long long lC[16][16];
+  const int t = threadIdx.x >> 4, c = threadIdx.x & 15;
+  const int b = blockIdx.x * 16 + c;
   double e = 0.0, z = 0.0;
-  unsigned long long c = 0;
-  for(int g = 0; g < groups; ++g) {
-    e += partial[(long)g * 2 * nbins + b];
-    z += partial[(long)g * 2 * nbins + nbins + b];
-    c += pcount[(long)g * nbins + b];
+  unsigned long long n = 0;
+  if(b < nbins)
+    for(int g = t; g < groups; g += 16) {
+      e += partial[(long)g * 2 * nbins + b];
+      z += partial[(long)g * 2 * nbins + nbins + b];
+      n += pcount[(long)g * nbins + b];
+    }
+  lE[t][c] = e; lZ[t][c] = z; lC[t][c] = n;
+  __syncthreads();
+  if(t == 0 && b < nbins) {
+    for(int k = 1; k < 16; ++k) { e += lE[k][c]; z += lZ[k][c]; n += lC[k][c]; }
+    sums[b] = e;
+    sums[nbins + b] = z;
+    counts[b] = n;
   }
-  sums[b] = e;
-  sums[nbins + b] = z;
-  counts[b] = c;
 }
 
 __global__ __launch_bounds__(256)
@@ -238,8 +249,8 @@ int spectral_shells(SpectralWork *W, const float *u, long s, int nv, double *E, 
   shell_kernel<<<W->grid, 256, lds, st>>>(W->z, W->n1, W->n2, nb, 1.0 / (n * n), W->partial,
                                           W->pcount);
   if(hipGetLastError() != hipSuccess) return SPECTRA_HIP;
-  shell_reduce_kernel<<<(nb + 255) / 256, 256, 0, st>>>(W->partial, W->pcount, W->grid, nb,
-                                                        W->sums, W->counts);
+  shell_reduce_kernel<<<(nb + 15) / 16, 256, 0, st>>>(W->partial, W->pcount, W->grid, nb,
+                                                      W->sums, W->counts);
   if(hipGetLastError() != hipSuccess) return SPECTRA_HIP;
   if(E && hipMemcpyAsync(E, W->sums, nb * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
     return SPECTRA_HIP;

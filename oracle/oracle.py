@@ -30,9 +30,11 @@ Pinning (which test fixes each function against something other than itself):
       (chain rule from (log r, theta) to (R, z))           tests/test_oracle_eqns.py
   first_order / Euler (src/*/Euler.cuh)
       hydro: identity.  disk: the quartic root x^4 + x - x0 = 0 by
-      substitution.  hydro-isot forcing: parity unpinned beyond the
-      zero-forcing identity (its random phases use libc rand() in the
-      reference, replaced here by SplitMix64).            tests/test_oracle_step.py
+      substitution.  hydro-isot forcing, for the phases drawn: kick along
+      (cos r1, sin r1), its value at the origin cell, exact sqrt(dt) and
+      1/rho scaling, the wavevector's truncation toward zero; the phase
+      stream itself is SplitMix64 in place of libc rand() (the reference's
+      own rand() sequence is not reproducible)         tests/test_oracle_step.py
   getdt closed form (src/getdt.cpp:26)
       closed form values                                  tests/test_oracle_step.py
   CFL time step (new; src/getdt.cpp:26-28 is a TODO)

@@ -210,3 +210,45 @@ def test_isot_random_phase_sequence():
                                    6.2831853071795865 * (b >> 11) * 2.0 ** -53], rtol=1e-15)
     # forcing off (f = 0): the state is untouched
     np.testing.assert_array_equal(o.download(), 0.0)
+
+
+def _forced(seed, dt, lnd=0.0, f=0.3, k=5.0, n1=24, n2=20):
+    o = Oracle("hydro-isot", n1, n2, params={"f": f, "k": k}, seed=seed)
+    x = np.zeros(o.shape)
+    x[..., 0] = lnd
+    o.upload(x)
+    o.euler(dt)
+    return o.download(), o.rand(), o.geometry()
+
+
+def test_isot_forcing_pinned():
+    """The stochastic forcing of src/hydro-isot/Euler.cuh:24-36, for the
+    phases (r1, r2) the oracle drew: a kick along the fixed direction
+    (cos r1, sin r1); sqrt(dt) cos(r2) cos(r1) f at the origin cell; exact
+    sqrt(dt) and 1/rho scaling; a plane wave whose integer wavevector
+    truncates k (cos r1, sin r1) + 1/2 toward zero, as the (Z) cast does."""
+    f, k, dt = 0.3, 5.0, 0.01
+    for seed in range(12):
+        y, (r1, r2), g = _forced(seed, dt, f=f, k=k)
+        du1, du2 = y[..., 1], y[..., 2]
+        # direction: du is parallel to (cos r1, sin r1)
+        np.testing.assert_allclose(du1 * np.sin(r1) - du2 * np.cos(r1), 0.0, atol=1e-15)
+        # origin cell: phi = 0
+        assert du1[0, 0] == pytest.approx(np.sqrt(dt) * f * np.cos(r2) * np.cos(r1), rel=1e-13, abs=1e-16)
+        # sqrt(dt) and 1/rho scaling
+        y4, _, _ = _forced(seed, 4 * dt, f=f, k=k)
+        np.testing.assert_allclose(y4[..., 1:], 2.0 * y[..., 1:], rtol=1e-14, atol=1e-17)
+        y2, _, _ = _forced(seed, dt, lnd=np.log(2.0), f=f, k=k)
+        np.testing.assert_allclose(y2[..., 1:], 0.5 * y[..., 1:], rtol=1e-13, atol=1e-17)
+        # wavevector: truncation toward zero of k n + 1/2
+        m1, m2 = k * np.cos(r1), k * np.sin(r1)
+        K1, K2 = int(m1 + 0.5), int(m2 + 0.5)             # Python int() truncates
+        i = np.arange(y.shape[0])[:, None]
+        j = np.arange(y.shape[1])[None, :]
+        phi = 2 * np.pi * (K1 * i * g["Delta1"] + K2 * j * g["Delta2"])
+        amp = np.sqrt(dt) * f * np.cos(phi + r2) / k
+        np.testing.assert_allclose(du1, amp * m1, rtol=1e-10, atol=1e-15)
+        if np.floor(m1 + 0.5) != K1 or np.floor(m2 + 0.5) != K2:
+            F1, F2 = int(np.floor(m1 + 0.5)), int(np.floor(m2 + 0.5))
+            phf = 2 * np.pi * (F1 * i * g["Delta1"] + F2 * j * g["Delta2"])
+            assert not np.allclose(du1, np.sqrt(dt) * f * np.cos(phf + r2) / k * m1, atol=1e-12)

@@ -141,6 +141,28 @@ def bench_batched(ev, steps=500, nruns=64, seed=7):
         assert r.check() == 0
         r.close()
 
+    # per-run launches of the plain path (bcond, kick, drift per substage, the
+    # reference's operation split), on a shorter sample of steps
+    runs = [fg2_amd.Run("hydro-isot", int(a), int(b), path=fg2_amd.PATH_PLAIN)
+            for a, b in zip(n1s, n2s)]
+    for r, x in zip(runs, xs):
+        r.upload(x)
+    plain_steps = 50
+    for r, d in zip(runs, dts):
+        r.step_async(0.0, d)
+    for r in runs:
+        ev.sync(r.stream())
+    t0 = time.perf_counter()
+    for k in range(plain_steps):
+        for r, d in zip(runs, dts):
+            r.step_async(k * d, d)
+    for r in runs:
+        ev.sync(r.stream())
+    t_opsplit = (time.perf_counter() - t0) * steps / plain_steps
+    for r in runs:
+        assert r.check() == 0
+        r.close()
+
     runs = [fg2_amd.Run("hydro-isot", int(a), int(b)) for a, b in zip(n1s, n2s)]
     for r, x in zip(runs, xs):
         r.upload(x)
@@ -160,6 +182,11 @@ def bench_batched(ev, steps=500, nruns=64, seed=7):
             "batched_cell_updates_per_s": cells * steps / t_batch,
             "per_run_cell_updates_per_s": cells * steps / t_plain,
             "speedup": t_plain / t_batch, "batched_s": t_batch, "per_run_s": t_plain,
+            "per_run_plain_path_cell_updates_per_s": cells * steps / t_opsplit,
+            "speedup_vs_plain_path": t_opsplit / t_batch,
+            "baselines": "per_run: each run's own one-pass step kernel, round-robin on the "
+                         "runs' streams; per_run_plain_path: each run's bcond/kick/drift "
+                         "launches (9 per step), timed over 50 steps and scaled",
             "tile": {"tile_h": tile[0], "waves": tile[1]}, "nan_runs": bad,
             "graph": "one hipGraph of all steps (graph build included in the timing)"}
 

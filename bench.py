@@ -254,12 +254,13 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=2, tile_
 def bench_disk(ev, n=2048, steps=200, warmup=10):
     """The disk module on its (log r, theta) grid, n x n, Neumann in r and
     reflecting in theta, so three substage launches per RK3 step: the
-    Keplerian disc of the precision study (Newtonian point mass, es = 1e6),
+    Keplerian disc of the precision study (point mass with softening length
+    0.05, es = 1e6),
     fixed dt = 0.9 x the FP64 CFL step of the initial state."""
     import fg2_amd
     from fg2_amd import ics
-    x = ics.disk_kepler(n, n, seed=4, h_over_r=0.05)
-    run = fg2_amd.Run("disk", n, n, params={"es": 1.0e6, "rS": 0.0})
+    x = ics.disk_kepler(n, n, seed=4, h_over_r=0.05, eps=0.05)
+    run = fg2_amd.Run("disk", n, n, params={"es": 1.0e6, "rS": 0.0, "eps": 0.05})
     run.upload(x)
     dt = 0.9 * run.getdt_cfl()
     stream = run.stream()
@@ -282,7 +283,7 @@ def bench_disk(ev, n=2048, steps=200, warmup=10):
     cells = n * n
     alg = 10 * 6 * 4 * cells
     launch_s = gpu_ms * 1e-3 / (3 * steps)
-    return {"workload": f"disk Keplerian disc {n}x{n}, Neumann r / reflecting theta, FP32, "
+    return {"workload": f"disk Keplerian disc {n}x{n} (softening 0.05), Neumann r / reflecting theta, FP32, "
                         "three substage launches per RK3 step, fixed 0.9 x CFL dt",
             "steps": steps, "dt": dt, "status": status,
             "ms_per_step": 1e3 * wall / steps,
@@ -371,6 +372,7 @@ def main():
     for k in range(args.warmup):
         run.step_async(k * dt, dt)
     assert run.check() == 0
+    sums0 = run.sums()                                  # FP64 mass and momentum
     e0, e1 = ev.event(), ev.event()
     ev.sync(stream)
     t0 = time.perf_counter()
@@ -385,6 +387,14 @@ def main():
     if status != 0:
         print("NaN in the benchmark state", file=sys.stderr)
         sys.exit(1)
+    sums1 = run.sums()
+    conservation = {"steps": args.steps,
+                    "mass_rel_drift": float(sums1[0] / sums0[0] - 1.0),
+                    "momentum1_drift_per_mass": float((sums1[1] - sums0[1]) / sums0[0]),
+                    "momentum2_drift_per_mass": float((sums1[2] - sums0[2]) / sums0[0]),
+                    "note": "FP64 sums over the grid of rho, rho u1, rho u2 before and after "
+                            "the timed steps; the scheme evolves ln rho, so mass is not "
+                            "conserved to round-off"}
 
     cells = N * N
     value = cells * args.steps / wall
@@ -438,6 +448,7 @@ def main():
                      "note": "achieved = algorithmic bytes (10 NVAR 4 B per cell) / launch "
                              "time; measured HBM traffic per launch is `traffic`"},
         "substage_path_ms_per_step": substage_ms,
+        "conservation": conservation,
         "batched": None,
         "serving": None,
         "disk_2048": None,

@@ -61,7 +61,8 @@ const std::vector<ParamDef> &param_defs(int m)
   static const std::vector<ParamDef> disk = {
     {"M", 1.0}, {"rS", 2.0}, {"rin", 3.0}, {"gamma", 5.0 / 3.0}, {"nus", 0.0}, {"nub", 0.0},
     {"kappa", 0.0}, {"alpha", 0.01}, {"ar", 1.0e8}, {"es", 1.0}, {"ff", 0.0}, {"dd", 1.0e-4},
-    {"nu", 2.0e-4}, {"ed", 5.0e-4}, {"rd", 5.0e-4}};
+    {"nu", 2.0e-4}, {"ed", 5.0e-4}, {"rd", 5.0e-4},
+    {"eps", 0.0}};   // gravity softening length: not in the reference, 0 keeps its force law
   return m == HYDRO ? hydro : m == HYDRO_ISOT ? isot : disk;
 }
 

@@ -61,7 +61,8 @@ struct Params {
   // hydro-isot first_order random phases (src/hydro-isot/Euler.cuh:24-25)
   float rand1, rand2;
   // disk: per-row table of the run, 4 floats per row i:
-  // {sphr = rin e^{(i+1/2) Delta1}, 1/sphr, sphr^2, M / (sphr - rS)^2}
+  // {sphr = rin e^{(i+1/2) Delta1}, 1/sphr, sphr^2, gravity M / (sphr - rS)^2
+  //  (softened when the parameter eps > 0)}
   const float *rowtab;
 };
 

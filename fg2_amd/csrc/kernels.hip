@@ -709,22 +709,31 @@ hipError_t launch_drift(int module, float *u, const float *v, long s, const Para
 
 // Per-row table of the disk grid (Params::rowtab): the row's spherical
 // radius (src/disk/eqns.cuh:52-54) and the gravity of the point mass
-// (src/disk/eqns.cuh:238-239), in the reference's expressions.
+// (src/disk/eqns.cuh:238-239), in the reference's expressions.  A softening
+// length eps > 0 (an added parameter; the reference has none) replaces
+// M / d^2 by the Plummer force M d / (d^2 + eps^2)^(3/2), d = sphr - rS.
 __global__ __launch_bounds__(256)
 void disk_rows_kernel(float *tab, const int n1, const float Delta1, const float M, const float rS,
-                      const float rin)
+                      const float rin, const float eps)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if(i >= n1) return;
   const float sphr = rin * expf(((float)i + 0.5f) * Delta1);
   const float tmp = sphr - rS;
-  reinterpret_cast<float4 *>(tab)[i] = make_float4(sphr, 1.0f / sphr, sphr * sphr, M / (tmp * tmp));
+  float gr;
+  if(eps == 0.0f) {
+    gr = M / (tmp * tmp);
+  } else {
+    const float d2 = tmp * tmp + eps * eps;
+    gr = M * tmp / (d2 * sqrtf(d2));
+  }
+  reinterpret_cast<float4 *>(tab)[i] = make_float4(sphr, 1.0f / sphr, sphr * sphr, gr);
 }
 
 hipError_t launch_disk_rows(float *tab, const Params &P, hipStream_t st)
 {
   disk_rows_kernel<<<(P.n1 + 255) / 256, 256, 0, st>>>(tab, P.n1, P.Delta1, P.par[0], P.par[1],
-                                                       P.par[2]);
+                                                       P.par[2], P.par[15]);
   return hipGetLastError();
 }
 

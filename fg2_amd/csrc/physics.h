@@ -200,7 +200,8 @@ struct HydroIsot {
 // ---------------------------------------------------------------------------
 // disk: (lnd, ur, uz, Omg, lne, lner) on the (log r, theta) plane;
 // src/disk/eqns.cuh:19-247, Euler.cuh:19-69.
-// par: M rS rin gamma nus nub kappa alpha ar es ff dd nu ed rd
+// par: M rS rin gamma nus nub kappa alpha ar es ff dd nu ed rd eps (eps: an
+// added gravity softening length, 0 = the reference's force law)
 // ---------------------------------------------------------------------------
 struct Disk {
   static constexpr int NV = 6, ODD = 1, ND = 4, FLOP = 733, ID = DISK;

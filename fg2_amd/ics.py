@@ -76,13 +76,14 @@ def isot_density(n1, n2, seed=3, kmax=8, rms=0.05, dtype=np.float64):
 
 
 def disk_kepler(n1, n2, seed=4, l1=None, rin=3.0, gm=1.0, h_over_r=0.05,
-                noise=0.01, floor=1e-3, rad_frac=1e-3, gamma=GAMMA):
+                noise=0.01, floor=1e-3, rad_frac=1e-3, gamma=GAMMA, eps=0.0):
     """disk: Keplerian disc around a central mass on the (log r, theta) grid
     of src/disk/config.h:21-34.  Sigma ~ R^-1 (rho_mid ~ R^-2) with a
     Gaussian vertical profile of aspect ratio h_over_r above a density floor
     of `floor` times the local midplane density, times (1 + noise N(0,1)).
-    Omega = sqrt(GM / r^3) on spheres (the Keplerian value in the midplane;
-    its centrifugal force balances the cylindrical-radial gravity
+    Omega = sqrt(GM / (r^2 + eps^2)^(3/2)) on spheres, the Keplerian value of
+    a point mass with Plummer softening length eps (the disk parameter eps;
+    the centrifugal force balances the cylindrical-radial gravity
     everywhere), isothermal sound speed h_over_r * v_K(R), ur = uz = 0, and
     radiation energy rad_frac times the thermal energy.  Returns (n1, n2, 6)."""
     rng = np.random.default_rng(seed)
@@ -103,7 +104,7 @@ def disk_kepler(n1, n2, seed=4, l1=None, rin=3.0, gm=1.0, h_over_r=0.05,
     e = cs2 / (gamma - 1.0)                      # (gamma - 1) e = c_s^2
     x = np.zeros((n1, n2, 6))
     x[..., 0] = np.log(rho)
-    x[..., 3] = np.sqrt(gm / sphr ** 3) * np.ones_like(R)
+    x[..., 3] = np.sqrt(gm / (sphr ** 2 + eps ** 2) ** 1.5) * np.ones_like(R)
     x[..., 4] = np.log(e)
     x[..., 5] = np.log(rad_frac * e)
     return x

@@ -59,7 +59,8 @@ std::vector<Param> default_params(int m)
   return {{"M", 1.0}, {"rS", 2.0}, {"rin", 3.0}, {"gamma", 5.0 / 3.0},
           {"nus", 0.0}, {"nub", 0.0}, {"kappa", 0.0}, {"alpha", 0.01},
           {"ar", 1.0e8}, {"es", 1.0}, {"ff", 0.0}, {"dd", 1.0e-4},
-          {"nu", 2.0e-4}, {"ed", 5.0e-4}, {"rd", 5.0e-4}};
+          {"nu", 2.0e-4}, {"ed", 5.0e-4}, {"rd", 5.0e-4},
+          {"eps", 0.0}};   // added softening length; 0 = the reference's gravity
 }
 
 // SplitMix64 (Steele, Lea, Flood 2014): the seeded stand-in for the libc
@@ -363,7 +364,9 @@ template<class R> void rhs_disk(const Sim<R> &S, const Stencil<R> &st,
   }
   {                                                                  // :238-244
     const R tmp = sphr - rS;
-    const R gr = M / (tmp * tmp);
+    const R eps = S.pr[15];   // added: Plummer softening M d / (d^2 + eps^2)^(3/2)
+    const R gr = eps == (R)0 ? M / (tmp * tmp)
+                             : M * tmp / ((tmp * tmp + eps * eps) * std::sqrt(tmp * tmp + eps * eps));
     dt[UR] -= sint * gr;
     dt[UZ] -= cost * gr;
   }

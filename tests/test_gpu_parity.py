@@ -114,6 +114,37 @@ def test_config1_hydro_256_100_steps(fg2):
     assert rho_err < 1e-6
 
 
+def test_config2_kh_3584_100_steps(fg2):
+    """The benchmark workload itself: hydro KH 3584^2 periodic, FP32, the
+    CFL dt of the initial state, 100 steps of the one-pass step kernel
+    against 100 steps of the FP64 oracle from the same FP32 state.  Bars:
+    lnd and lne within 1e-5 relative L2; the velocity vector (u1, u2) within
+    1e-5 relative L2 of its own norm.  u2 alone is 50x smaller than u1 (the
+    0.01 noise against the +-0.5 shear) and carries errors at the level of
+    u1's FP32 rounding, so its own-norm relative L2 is bounded at 2e-5
+    (measured 1.15e-5)."""
+    from oracle.oracle import Oracle
+    from fg2_amd import ics
+    n, steps = 3584, 100
+    _say(f"case config2 KH {n}^2 x {steps} (oracle: about 20 s on 16 threads)")
+    x = ics.hydro_kh(n, n, seed=2)
+    r = _run(fg2, "hydro", n, n, x, fg2.PATH_FUSED)
+    dt = r.getdt_cfl()
+    assert r.run(0.0, dt, steps) == 0
+    y = r.download()
+    r.close()
+    o = Oracle("hydro", n, n)
+    o.upload(x.astype(np.float32).astype(np.float64))
+    for k in range(0, steps, 20):
+        assert o.run(k * dt, dt, 20) == 0
+        _say(f"oracle {k + 20}/{steps}")
+    z = o.download()
+    e2 = rel_l2(y, z)
+    ev = np.sqrt(np.sum((y[..., 1:3] - z[..., 1:3]) ** 2) / np.sum(z[..., 1:3] ** 2))
+    _say("config2 rel L2 per field", e2, "Linf", rel_linf(y, z), "velocity", ev)
+    assert e2[0] < 1e-5 and e2[3] < 1e-5 and ev < 1e-5 and e2[1] < 1e-5 and e2[2] < 2e-5, (e2, ev)
+
+
 def test_nan_detection(fg2):
     """A NaN in the state makes step() return the NaN code (src/step.cpp:38-40)."""
     n = 64
@@ -330,5 +361,30 @@ def test_isot_forcing_vs_oracle(fg2):
     for k in range(steps):
         assert c.step(k * dt, dt) == 0
     assert np.max(np.abs(ya[..., 1:] - c.download()[..., 1:])) > 1e-3   # the forcing acted
+    err = rel_l2(ya, o.download())
+    assert np.all(err < 1e-5), err
+
+
+def test_disk_softened_vs_oracle(fg2):
+    """disk with the added gravity softening length eps > 0 (Plummer force
+    in the device row table): fused == plain bit for bit, and within 1e-5
+    relative L2 of the FP64 oracle after 5 steps."""
+    from oracle.oracle import Oracle
+    from fg2_amd import ics
+    n1, n2, steps = 48, 70, 5
+    params = dict(params_for("disk"), eps=0.3)
+    _say(f"case disk softened {n1}x{n2} steps={steps}")
+    x = ics.disk_kepler(n1, n2, seed=9, h_over_r=0.2, eps=0.3)
+    a = _run(fg2, "disk", n1, n2, x, fg2.PATH_FUSED, params=params)
+    b = _run(fg2, "disk", n1, n2, x, fg2.PATH_PLAIN, params=params)
+    o = Oracle("disk", n1, n2, params=params)
+    o.upload(x.astype(np.float32).astype(np.float64))
+    dt = 0.5 * o.getdt_cfl()
+    for k in range(steps):
+        assert a.step(k * dt, dt) == 0
+        assert b.step(k * dt, dt) == 0
+        assert o.step(k * dt, dt) == 0
+    ya = a.download()
+    np.testing.assert_array_equal(ya, b.download())
     err = rel_l2(ya, o.download())
     assert np.all(err < 1e-5), err

@@ -81,9 +81,15 @@ def test_transpose_symmetry_hydro():
                                rtol=1e-10, atol=1e-10)
 
 
-def test_disk_rotational_balance():
+@pytest.mark.parametrize("eps", [0.0, 0.4])
+def test_disk_rotational_balance(eps):
+    """At rest with Omega^2 R = g sin(theta), g = M d / (d^2 + eps^2)^(3/2)
+    and d = r - rS (eps = 0: the reference's M / d^2; eps > 0: the added
+    Plummer softening), the radial equation balances and the vertical one
+    feels gravity only."""
     n, M, rS = 16, 1.3, 0.5
-    o = Oracle("disk", n, n, params={"M": M, "rS": rS, "ff": 0.0, "alpha": 0.0, "nu": 0.0})
+    o = Oracle("disk", n, n, params={"M": M, "rS": rS, "ff": 0.0, "alpha": 0.0, "nu": 0.0,
+                                     "eps": eps})
     g = o.geometry()
     lnr = g["l1"] * (np.arange(n) + 0.5) / n
     th = g["l2"] * (np.arange(n) + 0.5) / n
@@ -91,7 +97,8 @@ def test_disk_rotational_balance():
     R = sphr * np.sin(th)[None, :]
     x = np.zeros(o.shape)
     x[..., 0], x[..., 4], x[..., 5] = -1.0, np.log(0.2), np.log(0.05)
-    grav = M / (sphr - rS) ** 2
+    d = sphr - rS
+    grav = M * d / (d * d + eps * eps) ** 1.5
     x[..., 3] = np.sqrt(grav * np.sin(th)[None, :] / R)
     o.upload(x)
     F = o.rhs()

@@ -310,6 +310,42 @@ def test_translation_invariance_full_size(fg2, module, n1, n2, shift):
     assert np.abs(ya - x).max() > 1e-6
 
 
+def test_translation_invariance_beyond_2_31_floats(fg2):
+    """A grid whose arrays exceed 2^31 floats (hydro 24576^2: 2.4e9 floats,
+    9.7 GB per array, 58 GB of HBM for the two runs): one step of a rolled
+    state equals the rolled step bit for bit, on the one-pass step kernel and
+    on the substage path, so no offset is computed in 32 bits.  The state is
+    a sum of seeded sinusoids (cheap to build at this size)."""
+    n = 24576
+    shift = (12345, 6789)
+    _say(f"case translation {n}x{n} (arrays beyond 2^31 floats)")
+    i = (np.arange(n, dtype=np.float32) / np.float32(n))[:, None]
+    j = (np.arange(n, dtype=np.float32) / np.float32(n))[None, :]
+    tp = np.float32(2 * np.pi)
+    x = np.empty((n, n, 4), np.float32)
+    x[..., 0] = 0.1 * np.sin(tp * (3 * i + 2 * j)) * np.cos(tp * 7 * j)
+    x[..., 1] = 0.05 * np.sin(tp * (5 * i - 3 * j)) + 0.01 * np.cos(tp * 29 * i)
+    x[..., 2] = 0.05 * np.cos(tp * (2 * i + 9 * j)) + 0.01 * np.sin(tp * 31 * j)
+    x[..., 3] = np.float32(np.log(0.9)) + 0.05 * np.sin(tp * (4 * i + j))
+    del i, j
+    xr = np.roll(x, shift, axis=(0, 1))
+    _say("state built")
+    for path in (fg2.PATH_FUSED, fg2.PATH_SUBSTAGE):
+        a = _run(fg2, "hydro", n, n, x, path)
+        dt = 0.5 * a.getdt()
+        assert a.step(0.0, dt) == 0
+        ya = a.download()
+        a.close()
+        b = _run(fg2, "hydro", n, n, xr, path)
+        assert b.step(0.0, dt) == 0
+        yb = b.download()
+        b.close()
+        _say(f"path {path} stepped")
+        assert np.array_equal(np.roll(ya, shift, axis=(0, 1)), yb)
+        assert np.abs(ya[::97, ::89] - x[::97, ::89]).max() > 1e-7      # the step moved the state
+        del ya, yb
+
+
 @pytest.mark.parametrize("n1,n2", [(3, 3), (3, 200), (200, 3), (7, 61), (65, 59), (1, 0)])
 def test_edge_shapes(fg2, n1, n2):
     """Smallest legal grids (3 cells per side), one-strip and ragged widths

@@ -295,14 +295,24 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
 #pragma unroll
   for(int k = 1; k <= 3; ++k) { nb[0][k - 1] = min(t + k, TW - 1); nb[1][k - 1] = max(t - k, 0); }
 
+  const int nit = (i1 - i0) + 4 * HALF;
+  // The right-most strip of a grid that the strips do not divide needs only
+  // lanes t <= n2 - j0 + 17 (its last output column plus the 9-column halo).
+  // Wavefronts wholly past that skip the work and keep only the workgroup's
+  // barriers, three per iteration as below.  The lanes next to them read
+  // stale LDS, which only reaches lanes more than 9 columns past the last
+  // output column.
+  if(64 * __builtin_amdgcn_readfirstlane(t >> 6) > n2 - j0 + 2 * H3 - 1) {
+    for(int it = 0; it < nit; ++it) { __syncthreads(); __syncthreads(); __syncthreads(); }
+    return;
+  }
+
   float W0[8][NV], W1[8][NV], W2[8][NV], V0[4][NV], V1[4][NV];
   auto load_row = [&](int r, float (&dst)[NV]) {
     load_cell<NV>(col + (long)pmod(r, n1) * s, dst);
   };
 #pragma unroll
   for(int k = 0; k < 7; ++k) load_row(i0 - H3 + k, W0[k]);
-
-  const int nit = (i1 - i0) + 4 * HALF;
 
   // one stage on a centre row whose window rows are win(-3..3)
   auto stage = [&](const int k, auto &&win, const int i, const float (&vprev)[NV],

@@ -74,9 +74,10 @@ def load_traffic():
     return d.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(budget_s=15.0):
+def cpu_baseline(budget_s=12.0):
     """The oracle's FP32 instantiation (OpenMP over rows, all host cores) on
-    the same 3584^2 KH workload, for as many steps as fit in budget_s."""
+    the same 3584^2 KH workload, for as many steps as fit in budget_s (about
+    12 s of host work)."""
     from oracle.oracle import Oracle
     from fg2_amd import ics
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -91,7 +92,7 @@ def cpu_baseline(budget_s=15.0):
         o.step(steps * dt, dt)
         steps += 1
         el = time.perf_counter() - t0
-        if el > budget_s or steps >= 50:
+        if el > budget_s or steps >= 1000:
             break
     rate = N * N * steps / el
     return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",

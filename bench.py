@@ -407,6 +407,14 @@ def main():
     achieved = alg_step / launch_s
     traffic = load_traffic()
     tile_h, waves = run.tile()
+    pm = run.perf_model()
+    # the reference's own perf model (src/config.cpp:30-36, printed per frame
+    # by src/solve.cpp:55-57) at the measured step time
+    ref_model = {"flop_per_step": pm["flops"], "bytes_per_step": pm["bits"] / 8,
+                 "gflops": pm["flops"] / (launch_s * 1e9),
+                 "gbytes_per_s": pm["bits"] / 8 / (launch_s * 1e9),
+                 "note": "eqns_flop + 2 NVAR flop per cell and substage, and 18 NVAR "
+                         "R-words per cell per step plus ghosts, as the reference counts"}
 
     # the same workload on the substage path (three launches per step), for reference
     run.set_path(fg2_amd.PATH_SUBSTAGE)
@@ -448,6 +456,7 @@ def main():
                      "avg_launch_us": launch_s * 1e6,
                      "note": "achieved = algorithmic bytes (10 NVAR 4 B per cell) / launch "
                              "time; measured HBM traffic per launch is `traffic`"},
+        "reference_perf_model": ref_model,
         "substage_path_ms_per_step": substage_ms,
         "conservation": conservation,
         "batched": None,

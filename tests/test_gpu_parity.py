@@ -424,3 +424,70 @@ def test_disk_softened_vs_oracle(fg2):
     np.testing.assert_array_equal(ya, b.download())
     err = rel_l2(ya, o.download())
     assert np.all(err < 1e-5), err
+
+
+def test_solve_equals_step_loop(fg2):
+    """fg2_solve (the inner loop of src/solve.cpp:37-49): with the closed-form
+    dt it takes the same steps as a host loop of fg2_step with
+    dt = min(t_final - t, getdt()), bit for bit, and ends exactly on t_final;
+    in CFL mode it follows the FP64 oracle's CFL-driven loop step for step
+    to within 1e-5 relative L2."""
+    from oracle.oracle import Oracle
+    n1, n2 = 64, 80
+    _say(f"case solve {n1}x{n2}")
+    x = state("hydro", n1, n2, seed=21)
+    a = _run(fg2, "hydro", n1, n2, x, fg2.PATH_FUSED)
+    b = _run(fg2, "hydro", n1, n2, x, fg2.PATH_FUSED)
+    T = 7.3 * a.getdt()
+    code, t, m = a.solve(0.0, T)
+    assert code == 0 and t == T and m == 8
+    tt, k = 0.0, 0
+    while tt < T:
+        dt = min(T - tt, b.getdt())
+        assert b.step(tt, dt) == 0
+        tt = min(tt + dt, T)
+        k += 1
+    assert k == m
+    np.testing.assert_array_equal(a.download(), b.download())
+    # CFL mode against the oracle's own CFL loop
+    c = _run(fg2, "hydro", n1, n2, x, fg2.PATH_FUSED)
+    c.set_dt_mode(fg2.DT_CFL)
+    o = Oracle("hydro", n1, n2)
+    o.upload(x.astype(np.float32).astype(np.float64))
+    T = 10.5 * o.getdt_cfl()
+    code, t, m = c.solve(0.0, T)
+    assert code == 0 and t == T
+    tt, k = 0.0, 0
+    while tt < T:
+        dt = min(T - tt, o.getdt_cfl())
+        assert o.step(tt, dt) == 0
+        tt = min(tt + dt, T)
+        k += 1
+    assert k == m
+    err = rel_l2(c.download(), o.download())
+    assert np.all(err < 1e-5), err
+
+
+def test_sums_vs_host(fg2):
+    """fg2_sums: FP64 mass and momentum (and the disk's angular momentum
+    with its (log r, theta) volume element) against the same sums of the
+    downloaded state on the host."""
+    for module in MODULES:
+        n1, n2 = 50, 70
+        x = state(module, n1, n2, seed=4)
+        r = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED)
+        got = r.sums()
+        y = r.download().astype(np.float64)
+        rho = np.exp(y[..., 0])
+        if module == "disk":
+            g = r.geometry()
+            lnr = (np.arange(n1) + 0.5) * np.float32(g["Delta1"])
+            th = (np.arange(n2) + 0.5) * np.float32(g["Delta2"])
+            sphr = 3.0 * np.exp(lnr)[:, None]
+            dV = sphr ** 3 * np.sin(th)[None, :]
+            R = sphr * np.sin(th)[None, :]
+            ref = [np.sum(rho * dV), np.sum(rho * y[..., 1] * dV), np.sum(rho * y[..., 2] * dV),
+                   np.sum(rho * y[..., 3] * R * R * dV)]
+        else:
+            ref = [np.sum(rho), np.sum(rho * y[..., 1]), np.sum(rho * y[..., 2]), 0.0]
+        np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9 * abs(ref[0]))

@@ -12,7 +12,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import params_for, rel_l2, rel_linf, state
+from helpers import DISK_PARAMS, params_for, rel_l2, rel_linf, state
 
 pytestmark = pytest.mark.gpu
 
@@ -491,3 +491,41 @@ def test_sums_vs_host(fg2):
         else:
             ref = [np.sum(rho), np.sum(rho * y[..., 1]), np.sum(rho * y[..., 2]), 0.0]
         np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9 * abs(ref[0]))
+
+
+NONDEFAULT = {
+    "hydro": {"dd": 2e-3, "nub": 3e-4, "kappa": 1e-3, "nus": 5e-4, "gamma": 1.4},
+    "hydro-isot": {"dd": 2e-3, "nub": 3e-4, "cs2": 0.7, "nus": 5e-4, "f": -0.05, "k": -2.0},
+    # free-free opacity on (ff > 0): the kappa / cooling terms and the
+    # radiation-equilibrium pre-step (src/disk/Euler.cuh) both run
+    "disk": dict(DISK_PARAMS, ff=1e-6, ar=1.0, dd=5e-4, nu=5e-4, ed=1e-3, rd=1e-3,
+                 alpha=0.02, nub=1e-4, kappa=1e-4, eps=0.1),
+}
+
+
+@pytest.mark.parametrize("module", MODULES)
+def test_nondefault_params_vs_oracle(fg2, module):
+    """The parameter-dependent branches the default parameters skip (density
+    diffusion dd, bulk viscosity nub, heat conduction, another gamma or cs2,
+    the Kolmogorov forcing, and for disk the free-free opacity with cooling
+    and the radiation-equilibrium pre-step): fused == plain bit for bit, and
+    FP32 within 1e-5 relative L2 of the FP64 oracle after 5 steps."""
+    from oracle.oracle import Oracle
+    n1, n2, steps = 48, 70, 5
+    params = NONDEFAULT[module]
+    _say(f"case non-default parameters {module}")
+    x = state(module, n1, n2, seed=7)
+    a = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, params=params)
+    b = _run(fg2, module, n1, n2, x, fg2.PATH_PLAIN, params=params)
+    o = Oracle(module, n1, n2, params=params)
+    o.upload(x.astype(np.float32).astype(np.float64))
+    dt = 0.25 * o.getdt_cfl()
+    for k in range(steps):
+        assert a.step(k * dt, dt) == 0
+        assert b.step(k * dt, dt) == 0
+        assert o.step(k * dt, dt) == 0
+    ya = a.download()
+    np.testing.assert_array_equal(ya, b.download())
+    err = rel_l2(ya, o.download())
+    _say(module, "rel L2", err)
+    assert np.all(err < 1e-5), err

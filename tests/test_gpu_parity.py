@@ -529,3 +529,64 @@ def test_nondefault_params_vs_oracle(fg2, module):
     err = rel_l2(ya, o.download())
     _say(module, "rel L2", err)
     assert np.all(err < 1e-5), err
+
+
+@pytest.mark.parametrize("module", ["hydro", "hydro-isot"])
+@pytest.mark.parametrize("p1,p2", [(1, 0), (0, 1)])
+def test_mixed_boundaries_vs_oracle(fg2, module, p1, p2):
+    """One periodic and one non-periodic dimension (Neumann in dim 1 or the
+    reflecting dim 2 with its odd variable): fused == plain bit for bit, and
+    within 1e-5 relative L2 of the FP64 oracle after 10 steps."""
+    from oracle.oracle import Oracle
+    n1, n2, steps = 56, 66, 10
+    _say(f"case mixed boundaries {module} p1={p1} p2={p2}")
+    geo = dict(l1=1.0, l2=0.8, p1=p1, p2=p2, c=0.5)
+    x = state(module, n1, n2, seed=13)
+    a = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, geo)
+    b = _run(fg2, module, n1, n2, x, fg2.PATH_PLAIN, geo)
+    o = Oracle(module, n1, n2, geometry=geo)
+    o.upload(x.astype(np.float32).astype(np.float64))
+    dt = 0.5 * a.getdt()
+    for k in range(steps):
+        assert a.step(k * dt, dt) == 0
+        assert b.step(k * dt, dt) == 0
+        assert o.step(k * dt, dt) == 0
+    ya = a.download()
+    np.testing.assert_array_equal(ya, b.download())
+    err = rel_l2(ya, o.download())
+    assert np.all(err < 1e-5), err
+
+
+def test_batch_substage_mode_equals_alone(fg2):
+    """Batches that take the substage kernels: disk runs of two sizes (each
+    with its own row table), and hydro runs of mixed periodicity.  Every run
+    gives the same bits in the batch as alone, eager and graph-replayed."""
+    cases = [("disk", [(48, 64), (80, 40)], [None, None]),
+             ("hydro", [(64, 70), (50, 90), (72, 72)],
+              [dict(l1=1.0, l2=1.0, p1=1, p2=1, c=0.5), dict(l1=1.0, l2=2.0, p1=0, p2=1, c=0.5),
+               dict(l1=1.5, l2=1.0, p1=0, p2=0, c=0.5)])]
+    for module, sizes, geos in cases:
+        _say(f"case batch substage mode {module}")
+        params = params_for(module)
+        xs = [state(module, a, b, seed=30 + k) for k, (a, b) in enumerate(sizes)]
+        alone = []
+        dts = []
+        for (a, b), x, g in zip(sizes, xs, geos):
+            r = _run(fg2, module, a, b, x, fg2.PATH_FUSED, g, params=params)
+            d = 0.25 * r.getdt_cfl()
+            dts.append(d)
+            assert r.run(0.0, d, 3) == 0
+            alone.append(r.download())
+        runs = [_run(fg2, module, a, b, x, fg2.PATH_FUSED, g, params=params)
+                for (a, b), x, g in zip(sizes, xs, geos)]
+        batch = fg2.Batch(runs, waves=2, tile_h=24)
+        for _ in range(3):
+            assert batch.step(dts) == 0
+        for r, y in zip(runs, alone):
+            np.testing.assert_array_equal(r.download(), y)
+        runs2 = [_run(fg2, module, a, b, x, fg2.PATH_FUSED, g, params=params)
+                 for (a, b), x, g in zip(sizes, xs, geos)]
+        b2 = fg2.Batch(runs2, waves=1, tile_h=16)
+        b2.graph_run(dts, steps=3, replays=1)
+        for r, y in zip(runs2, alone):
+            np.testing.assert_array_equal(r.download(), y)

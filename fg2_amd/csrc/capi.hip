@@ -923,6 +923,12 @@ struct fg2_batch {
   int *counters = nullptr;   // one per stage
   hipStream_t stream = nullptr;
   int parity = 0;
+  // hipGraph of the last fg2_batch_graph_run, kept for replays while its
+  // launch shape (steps, waves, grid, tiles) is unchanged; the launch records
+  // it reads are refilled before every call
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_steps = -1, g_waves = 0, g_grid = 0, g_total = 0;
 };
 
 namespace {
@@ -1060,6 +1066,8 @@ void fg2_batch_free(fg2_batch *b)
   for(int k = 0; k < 3; ++k)
     for(int p = 0; p < 2; ++p) if(b->dev[k][p]) (void)hipFree(b->dev[k][p]);
   if(b->counters) (void)hipFree(b->counters);
+  if(b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if(b->graph) (void)hipGraphDestroy(b->graph);
   if(b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -1075,9 +1083,27 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
   for(fg2_run *r : b->runs) HIP_TRY(hipStreamSynchronize(r->stream));
   std::vector<FusedShape> cand;
   const int hs[] = {16, 32, 48, 64, 96, 128};
-  const int ws_step[] = {4, 2, 1}, ws_sub[] = {4, 2, 1};
-  if(b->step) { for(int w : ws_step) for(int h : hs) cand.push_back({h, w}); }
-  else        { for(int w : ws_sub)  for(int h : hs) cand.push_back({h, w}); }
+  for(int w : {4, 2, 1}) {
+    for(int h : hs) cand.push_back({h, w});
+    // and the two heights the rounds model ranks first for this shape: the
+    // batch's tiles (every run's strips x chunks) over the resident slots
+    const int out = b->step ? 64 * w - 6 * HALF : STRIP;
+    const long slots = (long)batch_grid(b->module, b->step, w, b->cus) * (b->step ? 1 : w);
+    const long extra = b->step ? 6 : 3;
+    long c1 = -1, c2 = -1, h1 = 0, h2 = 0;
+    for(long h = 8; h <= 256; ++h) {
+      long tiles = 0;
+      for(fg2_run *r : b->runs)
+        tiles += (long)((r->n2 + out - 1) / out) * ((r->n1 + h - 1) / h);
+      const long cost = ((tiles + slots - 1) / slots) * (h + extra);
+      if(c1 < 0 || cost < c1) { c2 = c1; h2 = h1; c1 = cost; h1 = h; }
+      else if(c2 < 0 || cost < c2) { c2 = cost; h2 = h; }
+    }
+    for(long h : {h1, h2})
+      if(h > 0 && std::find_if(cand.begin(), cand.end(), [&](const FusedShape &c) {
+           return c.tile_h == h && c.waves == w; }) == cand.end())
+        cand.push_back({(int)h, w});
+  }
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
@@ -1148,37 +1174,34 @@ int fg2_batch_graph_run(fg2_batch *b, const double *dts, int steps, int replays)
     if(first_order_active(r))
       return fail(FG2_ERR_ARG, "batched runs with an active first_order are not supported");
   }
+  if((steps & 1) && replays > 1)
+    return fail(FG2_ERR_ARG, "replaying an odd step count more than once is not supported");
   if(int e = batch_fill(b, dts)) return e;
   HIP_TRY(hipStreamSynchronize(b->stream));
-  hipGraph_t g;
-  hipGraphExec_t ge;
-  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  int err = FG2_OK;
-  for(int k = 0; k < steps && !err; ++k) err = batch_launch(b, k & 1);
-  hipError_t ce = hipStreamEndCapture(b->stream, &g);
-  if(err) return err;
-  if(ce != hipSuccess) return fail(FG2_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
-  HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-  for(int k = 0; k < replays; ++k) {
-    HIP_TRY(hipGraphLaunch(ge, b->stream));
-    // a replay of an odd number of steps flips every run's state buffer
-    if(steps & 1) {
-      for(fg2_run *r : b->runs) r->cur ^= 1;
-      if(k + 1 < replays) {
-        HIP_TRY(hipStreamSynchronize(b->stream));
-        HIP_TRY(hipGraphExecDestroy(ge));
-        HIP_TRY(hipGraphDestroy(g));
-        return fail(FG2_ERR_ARG, "replaying an odd step count more than once is not supported");
-      }
-    }
+  const bool reuse = b->gexec && b->g_steps == steps && b->g_waves == b->waves &&
+                     b->g_grid == b->grid && b->g_total == b->total;
+  if(!reuse) {
+    if(b->gexec) { HIP_TRY(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
+    if(b->graph) { HIP_TRY(hipGraphDestroy(b->graph)); b->graph = nullptr; }
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int err = FG2_OK;
+    for(int k = 0; k < steps && !err; ++k) err = batch_launch(b, k & 1);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+    if(err) { if(g) (void)hipGraphDestroy(g); return err; }
+    if(ce != hipSuccess) return fail(FG2_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    b->g_steps = steps; b->g_waves = b->waves; b->g_grid = b->grid; b->g_total = b->total;
   }
+  for(int k = 0; k < replays; ++k) HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   for(fg2_run *r : b->runs) {
+    // a replay of an odd number of steps flips every run's state buffer
+    if(steps & 1) r->cur ^= 1;
     for(long k = 0; k < (long)steps * replays; ++k) randomize(r);
     r->lam_valid = false;
   }
-  HIP_TRY(hipGraphExecDestroy(ge));
-  HIP_TRY(hipGraphDestroy(g));
   return FG2_OK;
 }
 

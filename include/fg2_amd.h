@@ -228,8 +228,11 @@ int fg2_batch_create(fg2_batch **batch, fg2_run *const *runs, int nruns, int wav
 void fg2_batch_free(fg2_batch *batch);
 int fg2_batch_step_async(fg2_batch *batch, const double *dts);
 int fg2_batch_step(fg2_batch *batch, const double *dts);
-/* Capture `steps` batch steps with fixed per-run dts into a HIP graph and
- * replay it `replays` times. */
+/* Capture `steps` batch steps into a HIP graph and replay it `replays`
+ * times (an odd `steps` only once).  The graph is kept: a later call with
+ * the same `steps`, while the batch's workgroup shape and tiles are
+ * unchanged, replays it without a new capture.  The per-run launch records
+ * (state buffers, dts) are refilled on every call. */
 int fg2_batch_graph_run(fg2_batch *batch, const double *dts, int steps, int replays);
 void *fg2_batch_stream(fg2_batch *batch);
 /* Choose the batch's tile height and workgroup shape by timing batched

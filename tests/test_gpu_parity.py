@@ -197,13 +197,27 @@ def test_batch_equals_alone(fg2):
         assert batch.step(dts) == 0
     for r, y in zip(runs, alone):
         np.testing.assert_array_equal(r.download(), y)
-    # graph replay of the same two steps twice == 4 eager steps
+    # graph replay of the same two steps twice == 4 eager steps; a second
+    # call replays the kept graph (no new capture): 6 steps in all
     runs2 = [_run(fg2, "hydro-isot", a, b, x, fg2.PATH_FUSED) for (a, b), x in zip(sizes, xs)]
     b2 = fg2.Batch(runs2, waves=2, tile_h=32)
     b2.graph_run(dts, steps=2, replays=2)
     runs3 = [_run(fg2, "hydro-isot", a, b, x, fg2.PATH_FUSED, tile=(32, 1)) for (a, b), x in zip(sizes, xs)]
     for r, d in zip(runs3, dts):
         assert r.run(0.0, d, 4) == 0
+    for r, q in zip(runs2, runs3):
+        np.testing.assert_array_equal(r.download(), q.download())
+    b2.graph_run(dts, steps=2, replays=1)
+    for r, d in zip(runs3, dts):
+        assert r.run(0.0, d, 2) == 0
+    for r, q in zip(runs2, runs3):
+        np.testing.assert_array_equal(r.download(), q.download())
+    # an odd step count flips the state buffers; the next call refills the
+    # records for the flipped buffers and still matches
+    b2.graph_run(dts, steps=3, replays=1)
+    b2.graph_run(dts, steps=3, replays=1)
+    for r, d in zip(runs3, dts):
+        assert r.run(0.0, d, 6) == 0
     for r, q in zip(runs2, runs3):
         np.testing.assert_array_equal(r.download(), q.download())
 

@@ -1374,12 +1374,33 @@ int server_admit(fg2_server *S, double now)
   return FG2_OK;
 }
 
+// Tile height of one batched launch over the active runs of module m: the
+// server's fixed height, or (tile_h 0) the candidate that minimises the rounds
+// of resident workgroups times the cost of a tile (its rows + 6).
+int server_tile(const fg2_server *S, int m, int out)
+{
+  if(S->tile_h > 0) return S->tile_h;
+  const long slots = batch_grid(m, true, S->waves, S->cus);
+  int best_h = 64;
+  long best_cost = -1;
+  for(int h : {16, 24, 32, 40, 48, 56, 64, 72, 84, 96, 112, 128}) {
+    long tiles = 0;
+    for(const Request *q : S->active)
+      if(q->module == m && q->done_steps < q->steps)
+        tiles += (long)((q->n2 + out - 1) / out) * ((q->n1 + h - 1) / h);
+    const long cost = ((tiles + slots - 1) / slots) * (h + 6);
+    if(best_cost < 0 || cost < best_cost) { best_cost = cost; best_h = h; }
+  }
+  return best_h;
+}
+
 int server_launch(fg2_server *S)
 {
   const int out = 64 * S->waves - 6 * HALF;
   for(int m = 0; m < 2; ++m) {
     S->rec_host.clear();
     int begin = 0;
+    const int th = server_tile(S, m, out);
     for(Request *q : S->active) {
       if(q->module != m || q->done_steps >= q->steps) continue;
       BatchRun R{};
@@ -1387,7 +1408,7 @@ int server_launch(fg2_server *S)
       R.uout = q->in[q->cur ^ 1];
       R.s = q->s;
       R.P = q->P;
-      R.tile_h = std::min(S->tile_h, q->n1);
+      R.tile_h = std::min(th, q->n1);
       R.nstrips = (q->n2 + out - 1) / out;
       R.ntiles = R.nstrips * ((q->n1 + R.tile_h - 1) / R.tile_h);
       R.tile_begin = begin;
@@ -1476,7 +1497,7 @@ int fg2_server_create(fg2_server **out, size_t hbm_budget, int waves, int tile_h
   if(!S) return fail(FG2_ERR_NOMEM, "host allocation failed");
   S->budget = hbm_budget / 256 * 256;
   S->waves = waves;
-  S->tile_h = tile_h > 0 ? tile_h : 64;
+  S->tile_h = tile_h > 0 ? tile_h : 0;     // 0: chosen per launch (server_launch)
   S->ndiag = 8192;
   int dev = 0;
   hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);

@@ -194,7 +194,7 @@ def bench_batched(ev, steps=500, nruns=64, seed=7):
                      f"region and replayed {steps // gsteps} times"}
 
 
-def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=2, tile_h=0):
+def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_h=0):
     """Poisson stream of run requests (rate per second) mixing hydro and
     hydro-isot, sides log-uniform in 64..2048, steps uniform in 50..2000,
     seeded random-field initial states; continuous batching under an HBM
@@ -250,7 +250,8 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=2, tile_
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "cell_updates_per_s": cell_steps / wall,
             "wall_s": wall, "ticks": ticks, "host_s_in_tick": tick_s,
-            "hbm_budget_gib": budget_gib, "waves": waves,
+            "hbm_budget_gib": budget_gib,
+            "waves": waves if waves > 0 else "chosen per launch (rounds of resident workgroups)",
             "tile_h": tile_h if tile_h > 0 else "chosen per launch (rounds of resident workgroups)",
             "nan_requests": int(bad)}
 

@@ -423,7 +423,7 @@ class Batch:
 class Server:
     """Continuous-batching request loop over the C ABI (fg2_server_*)."""
 
-    def __init__(self, hbm_budget, waves=4, tile_h=0):
+    def __init__(self, hbm_budget, waves=0, tile_h=0):
         h = ctypes.c_void_p()
         _check(lib.fg2_server_create(ctypes.byref(h), int(hbm_budget), waves, tile_h))
         self.h = h

@@ -1374,33 +1374,38 @@ int server_admit(fg2_server *S, double now)
   return FG2_OK;
 }
 
-// Tile height of one batched launch over the active runs of module m: the
-// server's fixed height, or (tile_h 0) the candidate that minimises the rounds
-// of resident workgroups times the cost of a tile (its rows + 6).
-int server_tile(const fg2_server *S, int m, int out)
+// Workgroup shape and tile height of one batched launch over the active runs
+// of module m: the server's fixed values, or (0) the candidates that minimise
+// the rounds of resident workgroups times the cost of a tile (its rows + 6).
+FusedShape server_shape(const fg2_server *S, int m)
 {
-  if(S->tile_h > 0) return S->tile_h;
-  const long slots = batch_grid(m, true, S->waves, S->cus);
-  int best_h = 64;
+  FusedShape best{S->tile_h > 0 ? S->tile_h : 64, S->waves > 0 ? S->waves : 2};
+  if(S->tile_h > 0 && S->waves > 0) return best;
   long best_cost = -1;
-  for(int h : {16, 24, 32, 40, 48, 56, 64, 72, 84, 96, 112, 128}) {
-    long tiles = 0;
-    for(const Request *q : S->active)
-      if(q->module == m && q->done_steps < q->steps)
-        tiles += (long)((q->n2 + out - 1) / out) * ((q->n1 + h - 1) / h);
-    const long cost = ((tiles + slots - 1) / slots) * (h + 6);
-    if(best_cost < 0 || cost < best_cost) { best_cost = cost; best_h = h; }
+  for(int w : {4, 2, 1}) {
+    if(S->waves > 0 && w != S->waves) continue;
+    const int out = 64 * w - 6 * HALF;
+    const long slots = batch_grid(m, true, w, S->cus);
+    for(int h : {16, 24, 32, 40, 48, 56, 64, 72, 84, 96, 112, 128}) {
+      if(S->tile_h > 0 && h != S->tile_h) continue;
+      long tiles = 0;
+      for(const Request *q : S->active)
+        if(q->module == m && q->done_steps < q->steps)
+          tiles += (long)((q->n2 + out - 1) / out) * ((q->n1 + h - 1) / h);
+      const long cost = ((tiles + slots - 1) / slots) * (h + 6);
+      if(best_cost < 0 || cost < best_cost) { best_cost = cost; best = FusedShape{h, w}; }
+    }
   }
-  return best_h;
+  return best;
 }
 
 int server_launch(fg2_server *S)
 {
-  const int out = 64 * S->waves - 6 * HALF;
   for(int m = 0; m < 2; ++m) {
     S->rec_host.clear();
     int begin = 0;
-    const int th = server_tile(S, m, out);
+    const FusedShape sh = server_shape(S, m);
+    const int th = sh.tile_h, out = 64 * sh.waves - 6 * HALF;
     for(Request *q : S->active) {
       if(q->module != m || q->done_steps >= q->steps) continue;
       BatchRun R{};
@@ -1426,8 +1431,8 @@ int server_launch(fg2_server *S)
     HIP_TRY(hipMemcpyAsync(S->rec_dev[slot], S->rec_pinned[slot], S->rec_host.size() * sizeof(BatchRun),
                            hipMemcpyHostToDevice, S->stream));
     zero_ints_kernel<<<1, 64, 0, S->stream>>>(S->counters + slot, 1);
-    HIP_TRY(launch_batch_step(m, S->rec_dev[slot], (int)S->rec_host.size(), begin, S->waves,
-                              batch_grid(m, true, S->waves, S->cus), S->counters + slot, S->stream));
+    HIP_TRY(launch_batch_step(m, S->rec_dev[slot], (int)S->rec_host.size(), begin, sh.waves,
+                              batch_grid(m, true, sh.waves, S->cus), S->counters + slot, S->stream));
     HIP_TRY(hipEventRecord(S->rec_done[slot], S->stream));
   }
   for(Request *q : S->active) {
@@ -1492,7 +1497,8 @@ extern "C" {
 int fg2_server_create(fg2_server **out, size_t hbm_budget, int waves, int tile_h)
 {
   if(!out || hbm_budget < (1u << 20)) return fail(FG2_ERR_ARG, "bad server arguments");
-  if(waves != 1 && waves != 2 && waves != 4) return fail(FG2_ERR_SHAPE, "waves must be 1, 2 or 4");
+  if(waves != 0 && waves != 1 && waves != 2 && waves != 4)
+    return fail(FG2_ERR_SHAPE, "waves must be 0 (chosen per launch), 1, 2 or 4");
   fg2_server *S = new(std::nothrow) fg2_server;
   if(!S) return fail(FG2_ERR_NOMEM, "host allocation failed");
   S->budget = hbm_budget / 256 * 256;

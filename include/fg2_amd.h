@@ -248,8 +248,8 @@ typedef struct fg2_server fg2_server;
  * and advances all active runs together each tick (one batched one-pass
  * launch per module; served runs are periodic hydro or hydro-isot with the
  * module's default parameters).  waves, tile_h: the batched kernel's
- * workgroup and tile; tile_h 0 chooses the tile height per launch from the
- * active runs (rounds of resident workgroups times the tile's cost). */
+ * wavefronts per workgroup and tile height; 0 chooses it per launch from
+ * the active runs (rounds of resident workgroups times the tile's cost). */
 int fg2_server_create(fg2_server **server, size_t hbm_budget, int waves, int tile_h);
 void fg2_server_free(fg2_server *server);
 /* Queue a request of `steps` RK3 steps (closed-form dt per step) from the

@@ -255,16 +255,18 @@ def test_autotune_keeps_state_and_bits(fg2):
     np.testing.assert_array_equal(a.download(), b.download())
 
 
-def test_server_alone_vs_mix(fg2):
+@pytest.mark.parametrize("waves,tile_h", [(4, 48), (0, 0)])
+def test_server_alone_vs_mix(fg2, waves, tile_h):
     """Continuous batching: requests of both modules and different sizes and
     step counts, admitted under a tight HBM budget while others run, give
-    the same bits as each run stepped alone."""
+    the same bits as each run stepped alone; with a fixed launch shape and
+    with the shape chosen per launch (0, 0)."""
     import time
     reqs = [("hydro", 64, 96, 7), ("hydro-isot", 128, 64, 12), ("hydro", 100, 70, 3),
             ("hydro-isot", 70, 130, 9), ("hydro", 256, 256, 5), ("hydro-isot", 96, 96, 1)]
     xs = [state(m, a, b, seed=40 + k) for k, (m, a, b, _) in enumerate(reqs)]
     budget = 3 * 4 * 4 * 300 * 300 + (1 << 16)     # forces queueing
-    srv = fg2.Server(budget, waves=4, tile_h=48)
+    srv = fg2.Server(budget, waves=waves, tile_h=tile_h)
     ids = []
     t0 = time.perf_counter()
     for k, ((m, a, b, n), x) in enumerate(zip(reqs, xs)):

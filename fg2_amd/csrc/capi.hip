@@ -1221,8 +1221,10 @@ int fg2_batch_graph_run(fg2_batch *b, const double *dts, int steps, int replays)
 // State stays on the device from admission to retirement, in an arena
 // allocated once (first-fit with coalescing), so serving makes no hipMalloc
 // or hipFree calls; a served run holds only its two state arrays (the one-
-// pass step needs no v).  Launch records are double-buffered in pinned
-// memory behind events, so the host prepares tick k+1 while tick k runs.
+// pass step needs no v).  Launch records are quadruple-buffered in pinned
+// memory behind events, so the host prepares the next ticks while one runs.
+// Each launch's tile height and workgroup shape come from the active runs
+// (server_shape) unless fixed at creation.
 // A run's arithmetic is that of fg2_step (the same step tile), so a result
 // is the same bits alone and in any mix.
 // ---------------------------------------------------------------------------
@@ -1248,7 +1250,7 @@ struct Request {
 
 struct fg2_server {
   size_t budget = 0, in_use = 0;
-  int waves = 4, tile_h = 64, cus = 256;
+  int waves = 0, tile_h = 0, cus = 256;        // 0: chosen per launch (server_shape)
   hipStream_t stream = nullptr;
   char *arena = nullptr;
   std::map<size_t, size_t> free_list;          // offset -> size

@@ -188,3 +188,24 @@ def test_driver_parameters_and_crash_dump(fg2, tmp_path):
     assert "NaN" in r.stderr
     c = raw.read(tmp_path / "****.raw")
     assert c.time == 0.5 and c.shape == (32, 32, 4)
+
+
+def test_driver_options_reach_the_snapshot(fg2, tmp_path):
+    """The option parser of src/main.cpp:41-60: -b with two values (p = periodic,
+    anything else not), -c, -l with a time and two box sides, -n with frames and
+    two grid sides; the first frame's header carries them."""
+    if not os.path.exists(FG2RUN):
+        pytest.fail("fg2run is not built")
+    r = _fg2run(["hydro", "KH", "-b", "p", "n", "-c", "0.3", "-l", "0.001", "2", "3",
+                 "-n", "1", "32", "48"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    s = raw.read(tmp_path / "0000.raw")
+    assert s.shape == (32, 48, 4) and (s.p1, s.p2) == (1, 0)
+    assert (s.l1, s.l2) == (2.0, 3.0) and np.float32(s.c) == np.float32(0.3)
+    f = raw.read(tmp_path / "0001.raw")
+    assert f.time == pytest.approx(0.001)
+    # one -b value leaves p2 at its default (periodic), as the reference's parser does
+    r = _fg2run(["hydro", "-b", "n", "-n", "0", "16"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    s = raw.read(tmp_path / "0000.raw")
+    assert (s.p1, s.p2) == (0, 1)

@@ -1,7 +1,8 @@
 """Benchmark of the fused RK3 time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--no-batch] [--no-serve] [--no-disk] [--no-spectrum] [--disk-only]
+                    [--no-batch] [--no-serve] [--no-disk] [--no-spectrum] [--no-slab]
+                    [--disk-only] [--slab-only]
 
 Workload (the `value`): hydro module, Kelvin-Helmholtz shear layer on a
 3584 x 3584 periodic grid, FP32, one launch of the one-pass step kernel per
@@ -12,7 +13,8 @@ roofline of the step kernel, and beside it: the substage path on the same
 workload, 64 batched hydro-isot runs, a served request stream, the disk
 module on 2048 x 2048 (Neumann / reflecting boundaries, three substage
 launches per step), the energy and vorticity spectra of the 3584 x 3584
-state, and the host CPU baseline (the oracle's FP32
+state, the same grid cut into 8 slabs of 448 rows (one GPU's share, with
+its halo exchange), and the host CPU baseline (the oracle's FP32
 instantiation on all host cores, on a bounded sample of the same workload).
 """
 import argparse
@@ -304,6 +306,87 @@ def bench_disk(ev, n=2048, steps=200, warmup=10):
                           "ms_per_step": 1e3 * wall_t / steps}}
 
 
+def bench_slab(ev, dt, steps=200, warmup=10, nslabs=8, n=N):
+    """The hydro KH grid cut along dim 1 into nslabs slabs (one per GPU when
+    sharded), each stepped by the slab kernel from 9 halo rows per side.
+    share: one GPU's slab, (n / nslabs) x n, as a one-slab ring (its halos
+    are its own opposite edge, l1 scaled so Delta1 stays 1/n), per step one
+    halo exchange (2 x 9 rows of device copies) and one slab step, both on
+    the run's stream; whole: all nslabs slabs on this GPU, one exchange and
+    nslabs slab steps per grid step.  dt: the whole grid's CFL step."""
+    import fg2_amd
+    from fg2_amd import ics
+    h = n // nslabs
+    x = ics.hydro_kh(n, n, seed=2)
+
+    def timed(step, stream):
+        for k in range(warmup):
+            step()
+        ev.sync(stream)
+        e0, e1 = ev.event(), ev.event()
+        t0 = time.perf_counter()
+        ev.record(e0, stream)
+        for k in range(steps):
+            step()
+        ev.record(e1, stream)
+        ev.sync(stream)
+        return time.perf_counter() - t0, ev.elapsed_ms(e0, e1) / steps
+
+    share = fg2_amd.Run("hydro", h, n, geometry=dict(l1=h / n, l2=1.0, p1=1, p2=1, c=0.5))
+    share.slab(0, h)
+    share.upload(x[:h])
+    halo_bytes = share.slab_halo()["bytes"]
+
+    def share_step():
+        fg2_amd.slab_exchange([share])
+        share.slab_step(dt)
+
+    wall_s, ms_s = timed(share_step, share.stream())
+    _, ms_x = timed(lambda: fg2_amd.slab_exchange([share]), share.stream())
+    status = share.check()
+    tile_h, waves = share.tile()
+    share.close()
+
+    runs = []
+    for k in range(nslabs):
+        r = fg2_amd.Run("hydro", h, n, geometry=dict(l1=1.0, l2=1.0, p1=1, p2=1, c=0.5))
+        r.slab(k * h, n)
+        r.upload(x[k * h:(k + 1) * h])
+        runs.append(r)
+    def whole_step():
+        fg2_amd.slab_exchange(runs)
+        for r in runs:
+            r.slab_step(dt)
+
+    def sync_all():
+        for r in runs:
+            ev.sync(r.stream())
+
+    for k in range(warmup):
+        whole_step()
+    sync_all()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        whole_step()
+    sync_all()
+    wall_w = time.perf_counter() - t0
+    status = max(status, max(r.check() for r in runs))
+    for r in runs:
+        r.close()
+    cells = h * n
+    kernel_ms = ms_s - ms_x
+    return {"workload": f"hydro KH {n}x{n} cut into {nslabs} slabs of {h}x{n}; share = one "
+                        "slab with its 2 x 9 halo rows exchanged every step",
+            "steps": steps, "status": status,
+            "share": {"ms_per_step": ms_s, "kernel_ms_per_step": kernel_ms,
+                      "exchange_ms_per_step": ms_x, "exchange_bytes_per_step": 2 * halo_bytes,
+                      "cell_updates_per_s": cells / (ms_s * 1e-3),
+                      "hbm_frac_alg": cells * 10 * 4 * 4 / (kernel_ms * 1e-3) / HBM_PEAK,
+                      "tile": {"tile_h": tile_h, "waves": waves}},
+            "whole_on_one_gpu": {"ms_per_step": 1e3 * wall_w / steps,
+                                 "cell_updates_per_s": n * n * steps / wall_w}}
+
+
 def bench_spectrum(n=N, calls=10):
     """Energy and vorticity shell spectra (fg2_spectrum) of the hydro KH
     state on n x n: pack u1 + i u2, one FP32 complex 2-D FFT, FP64 shell sums;
@@ -357,8 +440,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-disk", action="store_true")
     ap.add_argument("--no-spectrum", action="store_true")
+    ap.add_argument("--no-slab", action="store_true")
     ap.add_argument("--disk-only", action="store_true",
                     help="run only the disk leg and print its JSON (profiling)")
+    ap.add_argument("--slab-only", action="store_true",
+                    help="run only the slab leg and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -378,6 +464,11 @@ def main():
     run.set_tile(args.tile, args.waves)
     run.upload(x)
     dt = run.getdt_cfl()
+    if args.slab_only:
+        run.close()
+        print(json.dumps({"slabs": bench_slab(ev, dt, steps=args.steps, warmup=args.warmup)}),
+              flush=True)
+        return
     stream = run.stream()
     for k in range(args.warmup):
         run.step_async(k * dt, dt)
@@ -472,6 +563,7 @@ def main():
         "serving": None,
         "disk_2048": None,
         "spectrum": None,
+        "slabs": None,
         "cpu_baseline": None,
     }
     run.close()
@@ -483,6 +575,10 @@ def main():
         out["disk_2048"] = bench_disk(ev)
     if not args.no_spectrum:
         out["spectrum"] = bench_spectrum()
+    if not args.no_slab:
+        out["slabs"] = bench_slab(ev, dt)
+        out["slabs"]["share"]["vs_whole_grid_share"] = \
+            ms_step / 8 / out["slabs"]["share"]["ms_per_step"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_baseline"]["config1_hydro_256"] = cpu_config1()

@@ -94,6 +94,10 @@ def _load():
         "fg2_substage": (I, [P, I, D]),
         "fg2_check": (I, [P]),
         "fg2_solve": (I, [P, DP, D, IP]),
+        "fg2_slab": (I, [P, I, I]),
+        "fg2_slab_halo": (I, [P, PP, PP, PP, PP, SP]),
+        "fg2_slab_exchange": (I, [PP, I]),
+        "fg2_slab_step": (I, [P, D]),
         "fg2_sums": (I, [P, DP]),
         "fg2_perf_model": (I, [P, DP, DP, DP]),
         "fg2_init": (I, [P, ctypes.c_char_p, ctypes.c_ulonglong]),
@@ -375,6 +379,31 @@ class Run:
 
     def stream(self):
         return lib.fg2_stream(self.h)
+
+    # -- slabs of a grid cut along dim 1 ----------------------------------
+    def slab(self, row0, n1_global):
+        """Make this run rows [row0, row0 + n1) of a periodic grid of
+        n1_global rows (0: a whole grid again)."""
+        _check(lib.fg2_slab(self.h, row0, n1_global))
+
+    def slab_halo(self):
+        """Device addresses of one halo exchange: recv_lo, recv_hi,
+        send_first, send_last and the byte count of each range."""
+        p = [ctypes.c_void_p() for _ in range(4)]
+        n = ctypes.c_size_t()
+        _check(lib.fg2_slab_halo(self.h, *[ctypes.byref(x) for x in p], ctypes.byref(n)))
+        return {"recv_lo": p[0].value, "recv_hi": p[1].value, "send_first": p[2].value,
+                "send_last": p[3].value, "bytes": n.value}
+
+    def slab_step(self, dt):
+        """One RK3 step of the slab from its current halos (asynchronous)."""
+        _check(lib.fg2_slab_step(self.h, dt))
+
+
+def slab_exchange(runs):
+    """Fill the halos of consecutive slabs of one grid from their neighbours."""
+    arr = (ctypes.c_void_p * len(runs))(*[r.h for r in runs])
+    _check(lib.fg2_slab_exchange(arr, len(runs)))
 
 
 class Batch:

@@ -131,6 +131,10 @@ struct fg2_run {
   char para_buf[256];
   size_t bytes = 0;
   int cus = 256;                 // compute units of the run's device
+  // slab of a taller periodic grid (slab_n1g = 0: the run is a whole grid)
+  int slab_row0 = 0, slab_n1g = 0;
+  float *slab_buf = nullptr;     // halo rows: SLAB_HALO above, then SLAB_HALO below, stride s
+  hipEvent_t slab_ev = nullptr;  // marks the state the neighbours copy from
 };
 
 namespace {
@@ -208,6 +212,18 @@ Params make_params(const fg2_run *r)
   return P;
 }
 
+static_assert(FG2_SLAB_HALO == SLAB_HALO, "halo rows of a slab step");
+
+// Rows of the grid along dim 1: the run's own, or the whole grid's for a slab.
+int rows_of_grid(const fg2_run *r) { return r->slab_n1g ? r->slab_n1g : r->n1; }
+
+// Halo rows of a slab for the step kernel (interior cell 0 of each buffer).
+SlabHalo slab_halo_of(const fg2_run *r)
+{
+  return SlabHalo{r->slab_buf + ROW_PAD, r->slab_buf + (long)SLAB_HALO * r->s + ROW_PAD,
+                  r->slab_row0, r->slab_n1g};
+}
+
 bool first_order_active(const fg2_run *r)
 {
   if(r->module == HYDRO_ISOT) return (float)r->par[4] * (float)r->par[5] > 0.0f;  // f k > 0
@@ -252,12 +268,15 @@ FusedShape step_shape_of(const fg2_run *r)
   if(sh.waves != 1 && sh.waves != 2 && sh.waves != 4) sh.waves = 4;
   if(r->tile_auto || sh.tile_h < 1) {
     // a workgroup tile of 64 NW - 18 output columns; stage 0 recomputes 12
-    // halo rows and stage 1 six, so a tile costs about its rows + 6
+    // halo rows and stage 1 six, so a tile costs about its rows + 6.  Tiles
+    // of at least 12 rows: below that the recomputed rows dominate (on a
+    // 448 x 3584 slab, 14-row tiles fill one round exactly and were the
+    // fastest measured, 7% ahead of 16)
     sh.waves = 4;
     const long out = 64 * sh.waves - 6 * HALF;
     const long nstrips = (r->n2 + out - 1) / out;
     const long slots = (long)blocks_per_cu(r->module, 1, sh.waves) * r->cus;
-    sh.tile_h = tile_rows(r->n1, nstrips, slots, 16, 6);
+    sh.tile_h = tile_rows(r->n1, nstrips, slots, 12, 6);
   }
   return sh;
 }
@@ -402,6 +421,8 @@ void fg2_free(fg2_run *r)
   if(r->sum_out) (void)hipFree(r->sum_out);
   if(r->rowtab) (void)hipFree(r->rowtab);
   if(r->spec) spectral_free(r->spec);
+  if(r->slab_buf) (void)hipFree(r->slab_buf);
+  if(r->slab_ev) (void)hipEventDestroy(r->slab_ev);
   if(r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
 }
@@ -440,8 +461,8 @@ int fg2_config(fg2_run *r)
     r->l1 = (float)(2.0 * hd1 * r->n1);
     r->p1 = 0; r->p2 = 0;
   }
-  // Delta = l / n in R (src/config.cpp:39)
-  r->Delta1 = (float)r->l1 / (float)r->n1;
+  // Delta = l / n in R (src/config.cpp:39); a slab takes the whole grid's
+  r->Delta1 = (float)r->l1 / (float)rows_of_grid(r);
   r->Delta2 = (float)r->l2 / (float)r->n2;
   r->configured = true;
   return refresh_rows(r);
@@ -627,7 +648,7 @@ double fg2_getdt(const fg2_run *r)
 {
   if(!r) return 0.0;
   // (E)c / (n1 / l1 + n2 / l2) with l, c in R (src/getdt.cpp:26)
-  const float a = (float)r->n1 / (float)r->l1, b = (float)r->n2 / (float)r->l2;
+  const float a = (float)rows_of_grid(r) / (float)r->l1, b = (float)r->n2 / (float)r->l2;
   return (double)(float)r->c / (double)(a + b);
 }
 
@@ -710,9 +731,10 @@ int time_launch(fg2_run *r, const FusedShape &sh, bool step, float *ms)
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
+  const SlabHalo H = slab_halo_of(r);
   auto once = [&]() -> hipError_t {
     if(step) return launch_step(r->module, r->in[r->cur], r->in[r->cur ^ 1], r->s, P, dt, sh, 0,
-                                r->diag, r->stream);
+                                r->diag, r->stream, r->slab_n1g ? &H : nullptr);
     for(int k = 0; k < 3; ++k) {
       const hipError_t e = launch_substage(r->module, k, r->in[r->cur], r->in[r->cur ^ 1], r->in[2],
                                            r->s, P, (float)RK_BETA[k], (float)(dt * RK_GAMMA[k]),
@@ -786,6 +808,7 @@ extern "C" int fg2_autotune(fg2_run *r, int *tile_h, int *waves)
 int fg2_step_async(fg2_run *r, double t, double dt)
 {
   if(int e = check_run(r)) return e;
+  if(r->slab_n1g) return fail(FG2_ERR_ARG, "a slab steps with fg2_slab_step");
   randomize(r);
   return substages(r, t, dt, r->dt_mode == FG2_DT_CFL);
 }
@@ -794,6 +817,7 @@ int fg2_substage(fg2_run *r, int stage, double dt)
 {
   if(int e = check_run(r)) return e;
   if(stage < 0 || stage > 2) return fail(FG2_ERR_ARG, "stage must be 0, 1 or 2");
+  if(r->slab_n1g) return fail(FG2_ERR_ARG, "a slab steps with fg2_slab_step");
   HIP_TRY(launch_substage(r->module, stage, r->in[r->cur], r->in[r->cur ^ 1], r->in[2], r->s,
                           make_params(r), (float)RK_BETA[stage], (float)(dt * RK_GAMMA[stage]),
                           shape_of(r), 0, r->diag, r->stream));
@@ -820,6 +844,105 @@ int fg2_step(fg2_run *r, double t, double dt)
 {
   if(int e = fg2_step_async(r, t, dt)) return e;
   return fg2_check(r);
+}
+
+// -- slabs of a grid cut along dim 1 -----------------------------------------
+
+int fg2_slab(fg2_run *r, int row0, int n1_global)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  if(n1_global == 0) {
+    r->slab_row0 = r->slab_n1g = 0;
+    return r->configured ? fg2_config(r) : FG2_OK;
+  }
+  if(r->module == DISK) return fail(FG2_ERR_ARG, "disk grids are not periodic along dim 1: no slabs");
+  if(r->n1 < SLAB_HALO)
+    return fail(FG2_ERR_SHAPE, "a slab needs at least %d rows, this run has %d", SLAB_HALO, r->n1);
+  if(n1_global < r->n1 || row0 < 0 || row0 >= n1_global)
+    return fail(FG2_ERR_SHAPE, "slab rows [%d, %d) do not fit a grid of %d rows", row0,
+                row0 + r->n1, n1_global);
+  if(!r->slab_buf) {
+    const size_t bytes = 2 * (size_t)SLAB_HALO * r->s * sizeof(float);
+    HIP_TRY(hipMalloc((void **)&r->slab_buf, bytes));
+    HIP_TRY(hipMemsetAsync(r->slab_buf, 0, bytes, r->stream));
+    HIP_TRY(hipEventCreateWithFlags(&r->slab_ev, hipEventDisableTiming));
+  }
+  r->slab_row0 = row0;
+  r->slab_n1g = n1_global;
+  r->lam_valid = false;
+  return r->configured ? fg2_config(r) : FG2_OK;
+}
+
+int fg2_slab_halo(fg2_run *r, float **recv_lo, float **recv_hi, const float **send_first,
+                  const float **send_last, size_t *bytes)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  if(!r->slab_n1g) return fail(FG2_ERR_ARG, "not a slab: call fg2_slab first");
+  const long rows = (long)SLAB_HALO * r->s;
+  // a row's storage starts ROW_PAD floats before its interior cell 0
+  if(recv_lo) *recv_lo = r->slab_buf;
+  if(recv_hi) *recv_hi = r->slab_buf + rows;
+  if(send_first) *send_first = r->in[r->cur] - ROW_PAD;
+  if(send_last) *send_last = r->in[r->cur] + (long)(r->n1 - SLAB_HALO) * r->s - ROW_PAD;
+  if(bytes) *bytes = (size_t)rows * sizeof(float);
+  return FG2_OK;
+}
+
+int fg2_slab_exchange(fg2_run *const *runs, int nslabs)
+{
+  if(!runs || nslabs < 1) return fail(FG2_ERR_ARG, "bad slab list");
+  const fg2_run *a = runs[0];
+  if(!a || !a->slab_n1g) return fail(FG2_ERR_ARG, "slab 0 is not a slab");
+  long rows = 0;
+  for(int k = 0; k < nslabs; ++k) {
+    const fg2_run *r = runs[k], *nx = runs[(k + 1) % nslabs];
+    if(!r || !r->slab_n1g) return fail(FG2_ERR_ARG, "slab %d is not a slab", k);
+    if(r->module != a->module || r->n2 != a->n2 || r->slab_n1g != a->slab_n1g)
+      return fail(FG2_ERR_ARG, "slab %d: module, n2 or grid rows differ from slab 0", k);
+    if((r->slab_row0 + r->n1) % r->slab_n1g != nx->slab_row0)
+      return fail(FG2_ERR_SHAPE, "slab %d ends at row %d but the next one starts at %d", k,
+                  r->slab_row0 + r->n1, nx->slab_row0);
+    rows += r->n1;
+  }
+  if(rows != a->slab_n1g)
+    return fail(FG2_ERR_SHAPE, "the slabs hold %ld rows, the grid %d", rows, a->slab_n1g);
+  for(int k = 0; k < nslabs; ++k) HIP_TRY(hipEventRecord(runs[k]->slab_ev, runs[k]->stream));
+  // Each slab pulls its halo rows on its own stream, after its own last step
+  // (which read the old halos) and after the neighbour's last step.
+  for(int k = 0; k < nslabs; ++k) {
+    fg2_run *r = runs[k];
+    const fg2_run *pv = runs[(k + nslabs - 1) % nslabs], *nx = runs[(k + 1) % nslabs];
+    float *lo = nullptr, *hi = nullptr;
+    const float *pv_last = nullptr, *nx_first = nullptr;
+    size_t bytes = 0;
+    fg2_slab_halo(r, &lo, &hi, nullptr, nullptr, &bytes);
+    fg2_slab_halo(const_cast<fg2_run *>(pv), nullptr, nullptr, nullptr, &pv_last, nullptr);
+    fg2_slab_halo(const_cast<fg2_run *>(nx), nullptr, nullptr, &nx_first, nullptr, nullptr);
+    HIP_TRY(hipStreamWaitEvent(r->stream, pv->slab_ev, 0));
+    HIP_TRY(hipMemcpyAsync(lo, pv_last, bytes, hipMemcpyDeviceToDevice, r->stream));
+    HIP_TRY(hipStreamWaitEvent(r->stream, nx->slab_ev, 0));
+    HIP_TRY(hipMemcpyAsync(hi, nx_first, bytes, hipMemcpyDeviceToDevice, r->stream));
+  }
+  return FG2_OK;
+}
+
+int fg2_slab_step(fg2_run *r, double dt)
+{
+  if(int e = check_run(r)) return e;
+  if(!r->slab_n1g) return fail(FG2_ERR_ARG, "not a slab: call fg2_slab first");
+  if(r->path != FG2_PATH_FUSED || !r->p1 || !r->p2)
+    return fail(FG2_ERR_ARG, "a slab steps with the one-pass kernel: fused path, periodic grid");
+  if(first_order_active(r))
+    return fail(FG2_ERR_ARG, "slabs with an active first_order are not supported");
+  randomize(r);
+  const int cfl = r->dt_mode == FG2_DT_CFL;
+  if(cfl) reset_diag_kernel<<<1, 1, 0, r->stream>>>(r->diag, 1);
+  const SlabHalo H = slab_halo_of(r);
+  HIP_TRY(launch_step(r->module, r->in[r->cur], r->in[r->cur ^ 1], r->s, make_params(r), dt,
+                      step_shape_of(r), cfl, r->diag, r->stream, &H));
+  r->cur ^= 1;
+  r->lam_valid = cfl != 0;
+  return FG2_OK;
 }
 
 int fg2_solve(fg2_run *r, double *t, double t_final, int *steps)
@@ -1037,6 +1160,7 @@ int fg2_batch_create(fg2_batch **out, fg2_run *const *runs, int nruns, int waves
   for(int i = 0; i < nruns; ++i) {
     if(int e = check_run(runs[i])) { delete b; return e; }
     if(runs[i]->module != b->module) { delete b; return fail(FG2_ERR_ARG, "a batch holds runs of one module"); }
+    if(runs[i]->slab_n1g) { delete b; return fail(FG2_ERR_ARG, "a batch holds whole grids, not slabs"); }
     b->step = b->step && runs[i]->p1 && runs[i]->p2 && runs[i]->module != DISK;
   }
   for(int i = 0; i < nruns; ++i) {

@@ -66,6 +66,19 @@ struct Params {
   const float *rowtab;
 };
 
+// A slab: rows [row0, row0 + n1) of a grid of n1g rows, periodic along dim
+// 1, whose other rows live in other runs (on this GPU or another).  The
+// one-pass step of a slab reads the SLAB_HALO = 9 rows on either side (three
+// substages of 3 rows) from two halo buffers in the state's own row layout:
+// lo points at interior cell (0,0) of global row row0 - 9, hi at that of
+// row row0 + n1 (rows wrap mod n1g), each followed by 8 more rows at the
+// run's row stride.
+constexpr int SLAB_HALO = 3 * HALF;
+struct SlabHalo {
+  const float *lo, *hi;
+  int row0, n1g;
+};
+
 // Diagnostics written by the kernels.  flags bit 0: a NaN was produced
 // (the sticky contain_nan of src/drift.cu:21,30); bit 1: an Inf was produced.
 struct DeviceDiag {

@@ -271,13 +271,16 @@ template<class M, int NW> struct StepLds {
 
 // One tile of the one-pass step: output rows [i0, i1), output columns
 // [j0, j0 + 64 NW - 18).  All threads of the workgroup call it together.
-template<class M, int NW>
+// SLAB: the run is rows [H.row0, H.row0 + n1) of a grid of H.n1g rows; rows
+// above and below it come from the halo buffers H.lo and H.hi instead of
+// wrapping, and the right-hand side sees the global row index.
+template<class M, int NW, bool SLAB = false>
 __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *__restrict__ uout,
                                           const long s, const Params &P, const float g0,
                                           const float g1, const float g2, const float b1,
                                           const float b2, const int j0, const int i0,
                                           const int i1, const int cfl, StepLds<M, NW> &L,
-                                          TileOut &res)
+                                          TileOut &res, const SlabHalo &H = SlabHalo{})
 {
   constexpr int NV = M::NV, ND = M::ND, TW = 64 * NW, H3 = 3 * HALF;
   auto &X = L.X;
@@ -309,7 +312,15 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
 
   float W0[8][NV], W1[8][NV], W2[8][NV], V0[4][NV], V1[4][NV];
   auto load_row = [&](int r, float (&dst)[NV]) {
-    load_cell<NV>(col + (long)pmod(r, n1) * s, dst);
+    if constexpr(SLAB) {
+      // r >= i0 - 9 >= -9 always; the last prefetch asks for row i1 + 9,
+      // which no stage reads, so it is clamped into the buffer
+      const float *base = r < 0 ? H.lo : r >= n1 ? H.hi : uin;
+      const int rr = r < 0 ? r + SLAB_HALO : r >= n1 ? min(r - n1, SLAB_HALO - 1) : r;
+      load_cell<NV>(base + (long)rr * s + (long)cc * NV, dst);
+    } else {
+      load_cell<NV>(col + (long)pmod(r, n1) * s, dst);
+    }
   };
 #pragma unroll
   for(int k = 0; k < 7; ++k) load_row(i0 - H3 + k, W0[k]);
@@ -345,7 +356,7 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
         x.mx[kk - 1][d] = G[k][kk - 1][d][nb[0][kk - 1]] - G[k][kk - 1][d][nb[1][kk - 1]];
     }
     float f[NV];
-    M::rhs(P, x, pmod(i, n1), cc, f);
+    M::rhs(P, x, SLAB ? pmod(H.row0 + i, H.n1g) : pmod(i, n1), cc, f);
 #pragma unroll
     for(int q = 0; q < NV; ++q) {
       vn[q] = (k == 0) ? f[q] : __builtin_fmaf(beta, vprev[q], f[q]);
@@ -411,6 +422,26 @@ void step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const 
   TileOut res{0, 0.0f, 0.0f};
   step_tile<M, NW>(uin, uout, s, P, g0, g1, g2, b1, b2, strip * (64 * NW - 6 * HALF), i0, i1,
                    cfl, L, res);
+  commit_tile(res, cfl, diag);
+}
+
+// The one-pass step of a slab: as step_kernel, with the rows outside the
+// slab read from its halo buffers (same per-cell arithmetic, so a grid cut
+// into slabs steps to the same bits as the whole grid).
+template<class M, int NW>
+__global__ __launch_bounds__(64 * NW)
+void slab_step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const long s,
+                      const Params P, const float g0, const float g1, const float g2,
+                      const float b1, const float b2, const int tile_h, const int nstrips,
+                      const int cfl, DeviceDiag *diag, const SlabHalo H)
+{
+  __shared__ StepLds<M, NW> L;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int strip = tile % nstrips, chunk = tile / nstrips;
+  const int i0 = chunk * tile_h, i1 = min(P.n1, i0 + tile_h);
+  TileOut res{0, 0.0f, 0.0f};
+  step_tile<M, NW, true>(uin, uout, s, P, g0, g1, g2, b1, b2, strip * (64 * NW - 6 * HALF), i0,
+                         i1, cfl, L, res, H);
   commit_tile(res, cfl, diag);
 }
 
@@ -807,9 +838,11 @@ hipError_t launch_cfl(int module, const float *u, long s, const Params &P, Devic
 }
 
 hipError_t launch_step(int module, const float *uin, float *uout, long s, const Params &P,
-                       double dt, const FusedShape &sh, int cfl, DeviceDiag *diag, hipStream_t st)
+                       double dt, const FusedShape &sh, int cfl, DeviceDiag *diag, hipStream_t st,
+                       const SlabHalo *slab)
 {
   if(!P.p1 || !P.p2) return hipErrorInvalidValue;
+  if(slab && (P.n1 < SLAB_HALO || !slab->lo || !slab->hi)) return hipErrorInvalidValue;
   const float g0 = (float)(dt * (1.0 / 3.0)), g1 = (float)(dt * (15.0 / 16.0)),
               g2 = (float)(dt * (8.0 / 15.0));
   const float b1 = (float)(-5.0 / 9.0), b2 = (float)(-153.0 / 128.0);
@@ -824,7 +857,11 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
 #define FG2_S(NW)                                                                            \
     case NW: {                                                                               \
       const int nstrips = (P.n2 + 64 * NW - 6 * HALF - 1) / (64 * NW - 6 * HALF);            \
-      step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(uin, uout, s, P, g0, g1, g2, \
+      if(slab)                                                                               \
+        slab_step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(                      \
+            uin, uout, s, P, g0, g1, g2, b1, b2, tile_h, nstrips, cfl, diag, *slab);           \
+      else                                                                                   \
+        step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(uin, uout, s, P, g0, g1, g2, \
                                                               b1, b2, tile_h, nstrips, cfl, diag); \
       break;                                                                                 \
     }

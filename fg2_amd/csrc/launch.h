@@ -32,10 +32,11 @@ hipError_t launch_substage(int module, int stage, const float *uin, float *uout,
 // Whole RK3 step in one launch (both dimensions periodic): the three
 // substages temporally blocked; reads uin, writes uout, v unused.  waves =
 // wavefronts per workgroup (1, 2 or 4); the workgroup recomputes 9 halo
-// columns on each side.
+// columns on each side.  With slab != NULL the run is a slab of a taller
+// periodic grid and the rows beyond it come from slab->lo / slab->hi.
 hipError_t launch_step(int module, const float *uin, float *uout, long s, const Params &P,
                        double dt, const FusedShape &shape, int cfl, DeviceDiag *diag,
-                       hipStream_t st);
+                       hipStream_t st, const SlabHalo *slab = nullptr);
 
 // Fill the disk row table (Params::rowtab, 4 floats per row) of a run
 // from its Delta1 and parameters M, rS, rin.

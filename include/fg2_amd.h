@@ -207,6 +207,38 @@ int fg2_check(fg2_run *run);
  * dumps it, src/solve.cpp:41-46). */
 int fg2_solve(fg2_run *run, double *t, double t_final, int *steps);
 
+/* ---- slabs: one grid cut along dim 1 over several runs (GPUs) ----
+ * No counterpart in the reference, which runs one grid on one GPU
+ * (src/setup.cpp:60-72).  For callers that shard a periodic hydro or
+ * hydro-isot grid by rows, one slab per GPU.  A slab's one-pass step reads
+ * FG2_SLAB_HALO rows on either side from two halo buffers the run owns;
+ * each step the caller fills them from the neighbouring slabs (RCCL
+ * send/recv, peer copies, or fg2_slab_exchange when all slabs live in one
+ * process).  A slab steps to the same bits as the whole grid. */
+#define FG2_SLAB_HALO 9
+/* Make the run rows [row0, row0 + n1) of a periodic grid of n1_global rows
+ * (n1_global = 0: a whole grid again).  Needs n1 >= FG2_SLAB_HALO.  After
+ * this, l1 (fg2_set_geometry) is the length of the whole grid, and getdt,
+ * Delta1 and the hydro-isot forcing use the whole grid's rows.  fg2_step,
+ * fg2_solve, fg2_substage and batches refuse a slab. */
+int fg2_slab(fg2_run *run, int row0, int n1_global);
+/* Device byte ranges of one halo exchange, each `bytes` long and contiguous
+ * (FG2_SLAB_HALO rows in the run's layout, row padding included):
+ * recv_lo <- the last rows of the slab above, recv_hi <- the first rows of
+ * the slab below; send_first / send_last are this slab's first / last rows
+ * in the current state (they move with every step).  Any may be NULL. */
+int fg2_slab_halo(fg2_run *run, float **recv_lo, float **recv_hi, const float **send_first,
+                  const float **send_last, size_t *bytes);
+/* Fill the halos of runs[0..nslabs) (consecutive slabs of one grid, in
+ * order, covering it; one device) from their neighbours with device copies
+ * on the receivers' streams, ordered after every slab's queued work. */
+int fg2_slab_exchange(fg2_run *const *runs, int nslabs);
+/* One RK3 step of size dt of a slab from its current halos, queued on the
+ * run's stream without a synchronisation (check NaN with fg2_check).  The
+ * CFL rates (FG2_DT_CFL) cover the slab's own cells: the grid's dt is the
+ * least over the slabs.  Hydro-isot stochastic forcing is refused. */
+int fg2_slab_step(fg2_run *run, double dt);
+
 /* FP64 diagnostics of the current state: out[0] = sum rho, out[1..2] =
  * sum rho u1, sum rho u2 (disk: rho ur, rho uz), out[3] = sum rho Omg R^2
  * (disk), each times the cell volume element (index-space for the

@@ -64,16 +64,20 @@ class HipEvents:
         assert self.hip.hipStreamSynchronize(ctypes.c_void_p(stream)) == 0
 
 
-def load_traffic():
-    """HBM bytes per launch of the time-step kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload scaled by the
-    copy-kernel calibration (None if the summary is absent)."""
+VALU_PEAK = 256 * 4 * 2.4e9 / 4   # wave64 FP32 VALU instructions/s: 4 cycles on a 16-lane SIMD
+
+
+def load_counters():
+    """HBM bytes and VALU wave-instructions per launch of the time-step
+    kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
+    the copy-kernel calibration) and SQ_INSTS_VALU counts of this workload
+    (None where the summary is absent)."""
     path = os.path.join(ROOT, "profiles", "step_kernel_hydro_3584_pmc.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         d = json.load(f)
-    return d.get("hbm_bytes_per_launch")
+    return d.get("hbm_bytes_per_launch"), d.get("valu_insts_per_launch")
 
 
 def cpu_baseline(budget_s=12.0):
@@ -505,7 +509,7 @@ def main():
     # periodic hydro runs take one launch of the one-pass step kernel per step
     launch_s = gpu_ms * 1e-3 / args.steps
     achieved = alg_step / launch_s
-    traffic = load_traffic()
+    traffic, valu_insts = load_counters()
     tile_h, waves = run.tile()
     pm = run.perf_model()
     # the reference's own perf model (src/config.cpp:30-36, printed per frame
@@ -555,7 +559,15 @@ def main():
                      "alg_bytes_per_launch": alg_step,
                      "avg_launch_us": launch_s * 1e6,
                      "note": "achieved = algorithmic bytes (10 NVAR 4 B per cell) / launch "
-                             "time; measured HBM traffic per launch is `traffic`"},
+                             "time; measured HBM traffic per launch is `traffic`",
+                     "valu": None if valu_insts is None else {
+                         "insts_per_launch": valu_insts,
+                         "achieved": valu_insts / launch_s / 1e9, "peak": VALU_PEAK / 1e9,
+                         "unit": "G wave-instructions/s",
+                         "frac": valu_insts / launch_s / VALU_PEAK,
+                         "note": "the bound that holds: SQ_INSTS_VALU per launch (counters) / "
+                                 "launch time, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles "
+                                 "per FP32 wave64 instruction (packed FP32 off)"}},
         "reference_perf_model": ref_model,
         "substage_path_ms_per_step": substage_ms,
         "conservation": conservation,

@@ -256,10 +256,10 @@ FusedShape shape_of(const fg2_run *r)
   return sh;
 }
 
-// The one-pass step kernel serves periodic runs of the Cartesian modules.
+// The one-pass step kernel serves the Cartesian modules, periodic or not.
 bool uses_step_kernel(const fg2_run *r)
 {
-  return r->path == FG2_PATH_FUSED && r->p1 && r->p2 && r->module != DISK;
+  return r->path == FG2_PATH_FUSED && r->module != DISK;
 }
 
 FusedShape step_shape_of(const fg2_run *r)
@@ -275,7 +275,8 @@ FusedShape step_shape_of(const fg2_run *r)
     sh.waves = 4;
     const long out = 64 * sh.waves - 6 * HALF;
     const long nstrips = (r->n2 + out - 1) / out;
-    const long slots = (long)blocks_per_cu(r->module, 1, sh.waves) * r->cus;
+    const int kind = (r->p1 && r->p2) ? 1 : 4;
+    const long slots = (long)blocks_per_cu(r->module, kind, sh.waves) * r->cus;
     sh.tile_h = tile_rows(r->n1, nstrips, slots, 12, 6);
   }
   return sh;
@@ -766,7 +767,7 @@ extern "C" int fg2_autotune(fg2_run *r, int *tile_h, int *waves)
   if(int e = check_run(r)) return e;
   if(r->path == FG2_PATH_PLAIN) return fail(FG2_ERR_ARG, "the plain path has no tile shape");
   const bool step = uses_step_kernel(r);
-  const TuneKey key{r->module, r->n1, r->n2, step ? 1 : 0};
+  const TuneKey key{r->module, r->n1, r->n2, step ? (r->p1 && r->p2 ? 1 : 2) : 0};
   FusedShape best{0, 0};
   {
     std::lock_guard<std::mutex> lk(g_tune_mutex);

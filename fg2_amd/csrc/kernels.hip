@@ -261,8 +261,53 @@ void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, fl
 // wrapped indices, so every halo value equals the value the substage path
 // computes at the wrapped cell, bit for bit.  Horizontal neighbours of a
 // stage's centre row come from LDS (one s_barrier per stage).
+//
+// Bounded runs (Neumann rows, p1 = 0, and / or reflecting columns, p2 = 0)
+// take the same schedule with the reference's ghost fill applied to every
+// stage's output instead of wrapping:
+//   rows: a stage output at row -1 - r (r = 0, 1, 2) is a copy of its output
+//     at row r, and at row n1 + k (k = 0, 1, 2) a copy of row n1 - 1 - k.
+//     The copies are made in the register rings the moment the source row
+//     is produced, which is before any stage reads them for a row that is
+//     kept (rows outside the grid feed only rows that are themselves
+//     replaced);
+//   columns: a lane reads its horizontal neighbours (and their g_k) at the
+//     mirrored column inside the grid, with the module's odd variable
+//     negated, so lanes outside the grid are never read.
+// Every value a kept cell sees is then the value of the substage path's
+// index-mapped read, bit for bit.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int pmod(int a, int n) { const int r = a % n; return r < 0 ? r + n : r; }
+// Mirror of an index about the faces of [0, n), clamped into [0, n) for
+// indices more than n outside (those feed only discarded values).
+__device__ __forceinline__ int mirror(int a, int n)
+{
+  const int m = a < 0 ? -1 - a : a >= n ? 2 * n - 1 - a : a;
+  return min(max(m, 0), n - 1);
+}
+
+enum StepMode { STEP_PERIODIC = 0, STEP_SLAB = 1, STEP_BOUNDED = 2 };
+
+// Neumann ghost rows of a stage output in a register ring of 8 rows where
+// row r (just produced) sits in slot S & 7 and row r - m in slot (S - m) & 7:
+// row r = 0, 1, 2 is copied to row -1 - r (2r + 1 slots back), and row
+// r = n1 + k (k = 0, 1, 2) is replaced by row n1 - 1 - k (2k + 1 slots back).
+// Called from the unrolled row loop, where S is a constant, so every ring
+// index folds to a register; the branches are uniform.
+template<int NV>
+__device__ __forceinline__ void ghost_rows(float (&W)[8][NV], const int S, const int r, const int n1)
+{
+  auto copy = [&](float (&dst)[NV], const float (&src)[NV]) {
+#pragma unroll
+    for(int q = 0; q < NV; ++q) dst[q] = src[q];
+  };
+  if(r == 0) copy(W[(S + 7) & 7], W[S & 7]);
+  else if(r == 1) copy(W[(S + 5) & 7], W[S & 7]);
+  else if(r == 2) copy(W[(S + 3) & 7], W[S & 7]);
+  if(r == n1) copy(W[S & 7], W[(S + 7) & 7]);
+  else if(r == n1 + 1) copy(W[S & 7], W[(S + 5) & 7]);
+  else if(r == n1 + 2) copy(W[S & 7], W[(S + 3) & 7]);
+}
 
 template<class M, int NW> struct StepLds {
   float X[3][M::NV][64 * NW];        // centre row of each stage
@@ -271,10 +316,11 @@ template<class M, int NW> struct StepLds {
 
 // One tile of the one-pass step: output rows [i0, i1), output columns
 // [j0, j0 + 64 NW - 18).  All threads of the workgroup call it together.
-// SLAB: the run is rows [H.row0, H.row0 + n1) of a grid of H.n1g rows; rows
-// above and below it come from the halo buffers H.lo and H.hi instead of
-// wrapping, and the right-hand side sees the global row index.
-template<class M, int NW, bool SLAB = false>
+// STEP_SLAB: the run is rows [H.row0, H.row0 + n1) of a grid of H.n1g rows;
+// rows above and below it come from the halo buffers H.lo and H.hi instead
+// of wrapping, and the right-hand side sees the global row index.
+// STEP_BOUNDED: P.p1 / P.p2 select wrapped or mirrored rows / columns.
+template<class M, int NW, int MODE = STEP_PERIODIC>
 __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *__restrict__ uout,
                                           const long s, const Params &P, const float g0,
                                           const float g1, const float g2, const float b1,
@@ -285,18 +331,35 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
   constexpr int NV = M::NV, ND = M::ND, TW = 64 * NW, H3 = 3 * HALF;
   auto &X = L.X;
   auto &G = L.G;
+  constexpr bool SLAB = MODE == STEP_SLAB, BOUNDED = MODE == STEP_BOUNDED;
   const int t = threadIdx.x;
   const int n1 = P.n1, n2 = P.n2;
   const int c = j0 - H3 + t;
-  const int cc = pmod(c, n2);                         // this lane's column
+  const bool mirror_cols = BOUNDED && !P.p2, mirror_rows = BOUNDED && !P.p1;
+  // this lane's column (lanes outside a reflecting grid load a mirrored
+  // column; no kept value reads them)
+  const int cc = mirror_cols ? mirror(c, n2) : pmod(c, n2);
   const bool out = t >= H3 && t < TW - H3 && c < n2;  // lanes of stage-2 output
   const float *col = uin + (long)cc * NV;
   const float k1a = 1.5859783962656393f / P.Delta1, k1b = 1.5859783962656393f / P.Delta2;
   // neighbour indices in LDS rows (clamped: lanes near the workgroup edge
-  // compute garbage that no output depends on)
+  // compute garbage that no output depends on); with reflecting columns the
+  // lane of the mirrored column, and the sign of the odd variable there
   int nb[2][3];
+  float sg[2][3];
 #pragma unroll
-  for(int k = 1; k <= 3; ++k) { nb[0][k - 1] = min(t + k, TW - 1); nb[1][k - 1] = max(t - k, 0); }
+  for(int k = 1; k <= 3; ++k) {
+    nb[0][k - 1] = min(t + k, TW - 1);
+    nb[1][k - 1] = max(t - k, 0);
+    sg[0][k - 1] = sg[1][k - 1] = 1.0f;
+    if(mirror_cols) {
+      const int cp = c + k, cm = c - k;
+      nb[0][k - 1] = min(max(mirror(cp, n2) - (j0 - H3), 0), TW - 1);
+      nb[1][k - 1] = min(max(mirror(cm, n2) - (j0 - H3), 0), TW - 1);
+      sg[0][k - 1] = (cp < 0 || cp >= n2) ? -1.0f : 1.0f;
+      sg[1][k - 1] = (cm < 0 || cm >= n2) ? -1.0f : 1.0f;
+    }
+  }
 
   const int nit = (i1 - i0) + 4 * HALF;
   // The right-most strip of a grid that the strips do not divide needs only
@@ -318,6 +381,8 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
       const float *base = r < 0 ? H.lo : r >= n1 ? H.hi : uin;
       const int rr = r < 0 ? r + SLAB_HALO : r >= n1 ? min(r - n1, SLAB_HALO - 1) : r;
       load_cell<NV>(base + (long)rr * s + (long)cc * NV, dst);
+    } else if constexpr(BOUNDED) {
+      load_cell<NV>(col + (long)(mirror_rows ? mirror(r, n1) : pmod(r, n1)) * s, dst);
     } else {
       load_cell<NV>(col + (long)pmod(r, n1) * s, dst);
     }
@@ -347,13 +412,17 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
         const float a = win(kk)[q], b = win(-kk)[q];
         x.dv[kk - 1][q] = a - b;
         x.sv[kk - 1][q] = a + b;
-        const float e = X[k][q][nb[0][kk - 1]], f = X[k][q][nb[1][kk - 1]];
+        float e = X[k][q][nb[0][kk - 1]], f = X[k][q][nb[1][kk - 1]];
+        if constexpr(BOUNDED) if(q == M::ODD) { e *= sg[0][kk - 1]; f *= sg[1][kk - 1]; }
         x.dh[kk - 1][q] = e - f;
         x.sh[kk - 1][q] = e + f;
       }
 #pragma unroll
-      for(int d = 0; d < ND; ++d)
-        x.mx[kk - 1][d] = G[k][kk - 1][d][nb[0][kk - 1]] - G[k][kk - 1][d][nb[1][kk - 1]];
+      for(int d = 0; d < ND; ++d) {
+        float e = G[k][kk - 1][d][nb[0][kk - 1]], f = G[k][kk - 1][d][nb[1][kk - 1]];
+        if constexpr(BOUNDED) if(M::D12V[d] == M::ODD) { e *= sg[0][kk - 1]; f *= sg[1][kk - 1]; }
+        x.mx[kk - 1][d] = e - f;
+      }
     }
     float f[NV];
     M::rhs(P, x, SLAB ? pmod(H.row0 + i, H.n1g) : pmod(i, n1), cc, f);
@@ -377,6 +446,9 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
               0.0f, g0, vn, uo);
 #pragma unroll
         for(int q = 0; q < NV; ++q) { W1[p & 7][q] = uo[q]; V0[p & 3][q] = vn[q]; }
+        // Neumann rows: the output of row r0 lives in slot p & 7, that of
+        // row r0 - m in slot (p - m) & 7
+        if constexpr(BOUNDED) if(mirror_rows) ghost_rows(W1, p, r0, n1);
       }
       if(it >= 2 * HALF) {                       // stage 1 on row r0 - 3
         float vn[NV], uo[NV];
@@ -384,6 +456,8 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
               V0[(p + 1) & 3], b1, g1, vn, uo);
 #pragma unroll
         for(int q = 0; q < NV; ++q) { W2[(p + 2) & 7][q] = uo[q]; V1[(p + 2) & 3][q] = vn[q]; }
+        // row r0 - 3 lives in slot (p + 2) & 7
+        if constexpr(BOUNDED) if(mirror_rows) ghost_rows(W2, p + 2, r0 - HALF, n1);
       } else {
         __syncthreads();
       }
@@ -440,8 +514,28 @@ void slab_step_kernel(const float *__restrict__ uin, float *__restrict__ uout, c
   const int strip = tile % nstrips, chunk = tile / nstrips;
   const int i0 = chunk * tile_h, i1 = min(P.n1, i0 + tile_h);
   TileOut res{0, 0.0f, 0.0f};
-  step_tile<M, NW, true>(uin, uout, s, P, g0, g1, g2, b1, b2, strip * (64 * NW - 6 * HALF), i0,
-                         i1, cfl, L, res, H);
+  step_tile<M, NW, STEP_SLAB>(uin, uout, s, P, g0, g1, g2, b1, b2, strip * (64 * NW - 6 * HALF),
+                              i0, i1, cfl, L, res, H);
+  commit_tile(res, cfl, diag);
+}
+
+// The one-pass step of a bounded run (Neumann rows and / or reflecting
+// columns): as step_kernel, with the reference's ghost fill applied to every
+// stage's output (same per-cell arithmetic as the substage path).
+template<class M, int NW>
+__global__ __launch_bounds__(64 * NW)
+void bounded_step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const long s,
+                         const Params P, const float g0, const float g1, const float g2,
+                         const float b1, const float b2, const int tile_h, const int nstrips,
+                         const int cfl, DeviceDiag *diag)
+{
+  __shared__ StepLds<M, NW> L;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int strip = tile % nstrips, chunk = tile / nstrips;
+  const int i0 = chunk * tile_h, i1 = min(P.n1, i0 + tile_h);
+  TileOut res{0, 0.0f, 0.0f};
+  step_tile<M, NW, STEP_BOUNDED>(uin, uout, s, P, g0, g1, g2, b1, b2,
+                                 strip * (64 * NW - 6 * HALF), i0, i1, cfl, L, res);
   commit_tile(res, cfl, diag);
 }
 
@@ -841,15 +935,15 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
                        double dt, const FusedShape &sh, int cfl, DeviceDiag *diag, hipStream_t st,
                        const SlabHalo *slab)
 {
-  if(!P.p1 || !P.p2) return hipErrorInvalidValue;
-  if(slab && (P.n1 < SLAB_HALO || !slab->lo || !slab->hi)) return hipErrorInvalidValue;
+  const bool bounded = !P.p1 || !P.p2;
+  if(slab && (bounded || P.n1 < SLAB_HALO || !slab->lo || !slab->hi)) return hipErrorInvalidValue;
   const float g0 = (float)(dt * (1.0 / 3.0)), g1 = (float)(dt * (15.0 / 16.0)),
               g2 = (float)(dt * (8.0 / 15.0));
   const float b1 = (float)(-5.0 / 9.0), b2 = (float)(-153.0 / 128.0);
   return by_module(module, [&](auto m) {
     using M = decltype(m);
     if constexpr(M::ID == DISK) {
-      return hipErrorInvalidValue;   // disk runs are non-periodic: substage path
+      return hipErrorInvalidValue;   // disk runs take the substage path
     } else {
     const int tile_h = std::max(1, sh.tile_h);
     const int nchunks = (P.n1 + tile_h - 1) / tile_h;
@@ -860,6 +954,9 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
       if(slab)                                                                               \
         slab_step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(                      \
             uin, uout, s, P, g0, g1, g2, b1, b2, tile_h, nstrips, cfl, diag, *slab);           \
+      else if(bounded)                                                                       \
+        bounded_step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(                   \
+            uin, uout, s, P, g0, g1, g2, b1, b2, tile_h, nstrips, cfl, diag);                  \
       else                                                                                   \
         step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(uin, uout, s, P, g0, g1, g2, \
                                                               b1, b2, tile_h, nstrips, cfl, diag); \
@@ -877,7 +974,8 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
 // Resident workgroups per CU of a kernel family (hardware occupancy from its
 // register and LDS use): kind 0 = the substage kernels (the least of the
 // three stages), kind 1 = the one-pass step kernel, kind 2 = the batched
-// substage kernels, kind 3 = the batched step kernel.
+// substage kernels, kind 3 = the batched step kernel, kind 4 = the one-pass
+// step kernel of bounded runs.
 hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
 {
   *blocks_per_cu = 0;
@@ -913,6 +1011,10 @@ hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
           if(waves == 1) take((const void *)step_kernel<M, 1>);
           if(waves == 2) take((const void *)step_kernel<M, 2>);
           if(waves == 4) take((const void *)step_kernel<M, 4>);
+        } else if(kind == 4) {
+          if(waves == 1) take((const void *)bounded_step_kernel<M, 1>);
+          if(waves == 2) take((const void *)bounded_step_kernel<M, 2>);
+          if(waves == 4) take((const void *)bounded_step_kernel<M, 4>);
         } else {
           if(waves == 1) take((const void *)batch_step_kernel<M, 1>);
           if(waves == 2) take((const void *)batch_step_kernel<M, 2>);

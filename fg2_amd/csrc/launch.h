@@ -29,11 +29,12 @@ hipError_t launch_substage(int module, int stage, const float *uin, float *uout,
                            const FusedShape &shape, int cfl, DeviceDiag *diag,
                            hipStream_t st);
 
-// Whole RK3 step in one launch (both dimensions periodic): the three
-// substages temporally blocked; reads uin, writes uout, v unused.  waves =
-// wavefronts per workgroup (1, 2 or 4); the workgroup recomputes 9 halo
-// columns on each side.  With slab != NULL the run is a slab of a taller
-// periodic grid and the rows beyond it come from slab->lo / slab->hi.
+// Whole RK3 step in one launch (hydro, hydro-isot): the three substages
+// temporally blocked; reads uin, writes uout, v unused.  waves = wavefronts
+// per workgroup (1, 2 or 4); the workgroup recomputes 9 halo columns on each
+// side.  Non-periodic dimensions (P.p1, P.p2) get the reference's ghost fill
+// at every stage.  With slab != NULL the run is a slab of a taller periodic
+// grid and the rows beyond it come from slab->lo / slab->hi.
 hipError_t launch_step(int module, const float *uin, float *uout, long s, const Params &P,
                        double dt, const FusedShape &shape, int cfl, DeviceDiag *diag,
                        hipStream_t st, const SlabHalo *slab = nullptr);
@@ -45,7 +46,8 @@ hipError_t launch_disk_rows(float *tab, const Params &P, hipStream_t st);
 // Resident workgroups per CU (hipOccupancyMaxActiveBlocksPerMultiprocessor)
 // with `waves` wavefronts per workgroup of the substage kernels (kind 0, the
 // least over the three stages), the one-pass step kernel (kind 1), the
-// batched substage kernels (kind 2) or the batched step kernel (kind 3).
+// batched substage kernels (kind 2), the batched step kernel (kind 3) or the
+// one-pass step kernel of bounded runs (kind 4).
 hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu);
 
 // CFL rates (FP64 max over cells of the advective and diffusive rates)

@@ -39,9 +39,9 @@ enum {
 /* Step implementation used by fg2_step / fg2_solve. */
 enum {
   FG2_PATH_FUSED    = 0, /* default: one launch per RK3 step (all three substages
-                            temporally blocked) when both dimensions are periodic
-                            and the module is hydro or hydro-isot; otherwise one
-                            fused kick+drift launch per substage                     */
+                            temporally blocked, any boundaries) for hydro and
+                            hydro-isot; one fused kick+drift launch per substage
+                            for disk                                                  */
   FG2_PATH_PLAIN    = 1, /* bcond, kick, drift launches per substage (src/step.cpp:30-35) */
   FG2_PATH_SUBSTAGE = 2  /* one fused kick+drift launch per substage, always         */
 };

@@ -224,20 +224,24 @@ def test_batch_equals_alone(fg2):
 
 @pytest.mark.parametrize("module", ["hydro", "hydro-isot"])
 @pytest.mark.parametrize("n1,n2", [(40, 50), (130, 300), (257, 251)])
-def test_step_kernel_equals_substages(fg2, module, n1, n2):
-    """The one-pass RK3 step (temporally blocked, periodic runs) gives the
-    bits of three substage launches, for any tile and workgroup shape,
-    including grids narrower than one workgroup strip (wrapped halos)."""
-    _say(f"case step kernel {module} {n1}x{n2}")
+@pytest.mark.parametrize("p1,p2", [(1, 1), (0, 0), (0, 1), (1, 0)])
+def test_step_kernel_equals_substages(fg2, module, n1, n2, p1, p2):
+    """The one-pass RK3 step (temporally blocked) gives the bits of three
+    substage launches, for any tile and workgroup shape, including grids
+    narrower than one workgroup strip (wrapped halos), and with Neumann rows
+    and / or reflecting columns (the ghost fill applied to every stage)."""
+    _say(f"case step kernel {module} {n1}x{n2} p1={p1} p2={p2}")
     x = state(module, n1, n2, seed=21)
-    ref = _run(fg2, module, n1, n2, x, fg2.PATH_SUBSTAGE)
+    geo = dict(l1=1.0, l2=1.0, p1=p1, p2=p2, c=0.5)
+    ref = _run(fg2, module, n1, n2, x, fg2.PATH_SUBSTAGE, geo)
     dt = 0.25 * ref.getdt()
     assert ref.run(0.0, dt, 3) == 0
     y = ref.download()
+    assert np.all(np.isfinite(y))
     for tile in [(0, 4), (8, 1), (37, 2), (128, 4), (300, 1)]:
-        r = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, tile=tile)
+        r = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, geo, tile=tile)
         assert r.run(0.0, dt, 3) == 0
-        np.testing.assert_array_equal(r.download(), y)
+        np.testing.assert_array_equal(r.download(), y, err_msg=f"tile {tile}")
 
 
 def test_autotune_keeps_state_and_bits(fg2):
@@ -363,11 +367,13 @@ def test_translation_invariance_beyond_2_31_floats(fg2):
 
 
 @pytest.mark.parametrize("n1,n2", [(3, 3), (3, 200), (200, 3), (7, 61), (65, 59), (1, 0)])
-def test_edge_shapes(fg2, n1, n2):
+@pytest.mark.parametrize("per", [1, 0])
+def test_edge_shapes(fg2, n1, n2, per):
     """Smallest legal grids (3 cells per side), one-strip and ragged widths
-    across every path: fused == plain bit for bit; sides below 3 are refused
-    with FG2_ERR_SHAPE (the reference's ghost fill needs 3 cells)."""
-    _say(f"case edge shape {n1}x{n2}")
+    across every path, periodic and bounded: fused == plain bit for bit;
+    sides below 3 are refused with FG2_ERR_SHAPE (the reference's ghost fill
+    needs 3 cells)."""
+    _say(f"case edge shape {n1}x{n2} periodic={per}")
     if min(n1, n2) < 3:
         with pytest.raises(fg2.Fg2Error) as e:
             fg2.Run("hydro", max(n1, 1), max(n2, 1))
@@ -375,8 +381,9 @@ def test_edge_shapes(fg2, n1, n2):
         return
     x = state("hydro", n1, n2, seed=9)
     outs = []
+    geo = dict(l1=1.0, l2=1.0, p1=per, p2=per, c=0.5)
     for path in (fg2.PATH_PLAIN, fg2.PATH_SUBSTAGE, fg2.PATH_FUSED):
-        r = _run(fg2, "hydro", n1, n2, x, path)
+        r = _run(fg2, "hydro", n1, n2, x, path, geo)
         dt = 0.2 * r.getdt()
         assert r.run(0.0, dt, 3) == 0
         outs.append(r.download())

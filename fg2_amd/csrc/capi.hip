@@ -1206,6 +1206,10 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
 {
   if(!b || !dts) return fail(FG2_ERR_ARG, "NULL argument");
   for(fg2_run *r : b->runs) HIP_TRY(hipStreamSynchronize(r->stream));
+  // A candidate is a tile height h and a width; each is timed twice: every
+  // run cut into h-row tiles (the last one shorter), and every run cut into
+  // the same number of tiles of equal height (at most h rows), which keeps
+  // short last tiles from idling their slots for most of the launch.
   std::vector<FusedShape> cand;
   const int hs[] = {16, 32, 48, 64, 96, 128};
   for(int w : {4, 2, 1}) {
@@ -1229,15 +1233,23 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
            return c.tile_h == h && c.waves == w; }) == cand.end())
         cand.push_back({(int)h, w});
   }
+  auto height = [&](int h, bool even, int i) {
+    const int n1 = b->runs[i]->n1;
+    if(!even) return std::min(h, n1);
+    const int chunks = (n1 + h - 1) / h;
+    return (n1 + chunks - 1) / chunks;
+  };
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   float best_ms = 1e30f;
   FusedShape best{0, 0};
-  for(const FusedShape &c : cand) {
+  bool best_even = false;
+  for(const FusedShape &c : cand)
+  for(bool even : {false, true}) {
     b->waves = c.waves;
     b->grid = batch_grid(b->module, b->step, c.waves, b->cus);
-    for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = std::min(c.tile_h, b->runs[i]->n1);
+    for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = height(c.tile_h, even, i);
     HIP_TRY(hipStreamSynchronize(b->stream));
     if(int e = batch_fill(b, dts)) return e;
     if(int e = batch_launch(b, 0)) return e;
@@ -1251,13 +1263,13 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
       HIP_TRY(hipEventElapsedTime(&t, e0, e1));
       ms = std::min(ms, t);
     }
-    if(ms < best_ms) { best_ms = ms; best = c; }
+    if(ms < best_ms) { best_ms = ms; best = c; best_even = even; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->waves = best.waves;
   b->grid = batch_grid(b->module, b->step, best.waves, b->cus);
-  for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = std::min(best.tile_h, b->runs[i]->n1);
+  for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = height(best.tile_h, best_even, i);
   for(fg2_run *r : b->runs) reset_diag_kernel<<<1, 1, 0, b->stream>>>(r->diag, 0);
   HIP_TRY(hipStreamSynchronize(b->stream));
   if(tile_h) *tile_h = best.tile_h;

@@ -268,7 +268,9 @@ int fg2_batch_step(fg2_batch *batch, const double *dts);
 int fg2_batch_graph_run(fg2_batch *batch, const double *dts, int steps, int replays);
 void *fg2_batch_stream(fg2_batch *batch);
 /* Choose the batch's tile height and workgroup shape by timing batched
- * steps of size dts on the runs' states (states unchanged). */
+ * steps of size dts on the runs' states (states unchanged).  Each height h
+ * is timed with every run cut into h-row tiles and with every run cut into
+ * the same number of equal tiles; *tile_h receives h. */
 int fg2_batch_autotune(fg2_batch *batch, const double *dts, int *tile_h, int *waves);
 
 /* ---- serving: continuous batching of runs of different sizes ---- */

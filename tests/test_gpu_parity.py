@@ -330,6 +330,42 @@ def test_translation_invariance_full_size(fg2, module, n1, n2, shift):
     assert np.abs(ya - x).max() > 1e-6
 
 
+@pytest.mark.parametrize("module,n1,n2,axis", [
+    ("hydro", 3584, 3584, 0),                # the benchmark shape, Neumann rows
+    ("hydro", 3584, 3584, 1),                # reflecting columns
+    ("hydro-isot", 1000, 1536, 0),
+])
+def test_mirror_symmetry_bounded_full_size(fg2, module, n1, n2, axis):
+    """Bounded runs (Neumann rows, reflecting columns) are mirror symmetric
+    bit for bit: flipping the state along dim 1 (dim 2) with u1 (u2)
+    negated gives the flipped, negated result, because every operation of
+    the scheme commutes exactly with the negation and both faces of a
+    dimension take the same ghost fill.  A size-independent check of the
+    one-pass kernel's ghost rows and mirrored columns at full size; a top /
+    bottom or left / right asymmetry in them would break it."""
+    from fg2_amd import ics
+    _say(f"case mirror symmetry {module} {n1}x{n2} axis={axis}")
+    if module == "hydro":
+        x = ics.hydro_kh(n1, n2, seed=4).astype(np.float32)
+    else:
+        x = state(module, n1, n2, seed=4).astype(np.float32)
+
+    def flip(y):
+        y = np.flip(y, axis=axis).copy()
+        y[..., 1 + axis] = -y[..., 1 + axis]
+        return y
+
+    geo = dict(l1=1.0, l2=1.0, p1=0, p2=0, c=0.5)
+    a = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, geo)
+    b = _run(fg2, module, n1, n2, flip(x), fg2.PATH_FUSED, geo)
+    dt = a.getdt_cfl()
+    assert a.run(0.0, dt, 10) == 0 and b.run(0.0, dt, 10) == 0
+    ya, yb = a.download(), b.download()
+    assert np.all(np.isfinite(ya))
+    np.testing.assert_array_equal(flip(ya), yb)
+    assert np.abs(ya - x).max() > 1e-6
+
+
 def test_translation_invariance_beyond_2_31_floats(fg2):
     """A grid whose arrays exceed 2^31 floats (hydro 24576^2: 2.4e9 floats,
     9.7 GB per array, 58 GB of HBM for the two runs): one step of a rolled

@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--no-batch] [--no-serve] [--no-disk] [--no-spectrum] [--no-slab]
-                    [--disk-only] [--slab-only]
+                    [--no-bounded] [--disk-only] [--slab-only] [--bounded-only]
 
 Workload (the `value`): hydro module, Kelvin-Helmholtz shear layer on a
 3584 x 3584 periodic grid, FP32, one launch of the one-pass step kernel per
@@ -13,8 +13,9 @@ roofline of the step kernel, and beside it: the substage path on the same
 workload, 64 batched hydro-isot runs, a served request stream, the disk
 module on 2048 x 2048 (Neumann / reflecting boundaries, three substage
 launches per step), the energy and vorticity spectra of the 3584 x 3584
-state, the same grid cut into 8 slabs of 448 rows (one GPU's share, with
-its halo exchange), and the host CPU baseline (the oracle's FP32
+state, the KH grid with Neumann rows and reflecting columns, the same grid
+cut into 8 slabs of 448 rows (one GPU's share, with its halo exchange), and
+the host CPU baseline (the oracle's FP32
 instantiation on all host cores, on a bounded sample of the same workload).
 """
 import argparse
@@ -391,6 +392,39 @@ def bench_slab(ev, dt, steps=200, warmup=10, nslabs=8, n=N):
                                  "cell_updates_per_s": n * n * steps / wall_w}}
 
 
+def bench_bounded(ev, steps=200, warmup=10, n=N):
+    """The KH workload with Neumann rows and reflecting columns (p1 = p2 = 0):
+    the one-pass step kernel with the ghost fill applied to every stage
+    (bounded_step_kernel), against the substage path on the same state.
+    Fixed dt = the FP64 CFL step of the initial state."""
+    import fg2_amd
+    from fg2_amd import ics
+    x = ics.hydro_kh(n, n, seed=2)
+    out = {"workload": f"hydro KH {n}x{n}, Neumann rows and reflecting columns, FP32, "
+                       "fixed CFL dt", "steps": steps}
+    for name, path in (("one_pass", fg2_amd.PATH_FUSED), ("substage", fg2_amd.PATH_SUBSTAGE)):
+        run = fg2_amd.Run("hydro", n, n, geometry=dict(l1=1.0, l2=1.0, p1=0, p2=0, c=0.5),
+                          path=path)
+        run.upload(x)
+        dt = run.getdt_cfl()
+        stream = run.stream()
+        for k in range(warmup):
+            run.step_async(k * dt, dt)
+        ev.sync(stream)
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0, stream)
+        for k in range(steps):
+            run.step_async(k * dt, dt)
+        ev.record(e1, stream)
+        ms = ev.elapsed_ms(e0, e1) / steps
+        tile_h, waves = run.tile()
+        out[name] = {"ms_per_step": ms, "cell_updates_per_s": n * n / (ms * 1e-3),
+                     "status": run.check(), "tile": {"tile_h": tile_h, "waves": waves}}
+        run.close()
+    out["speedup"] = out["substage"]["ms_per_step"] / out["one_pass"]["ms_per_step"]
+    return out
+
+
 def bench_spectrum(n=N, calls=10):
     """Energy and vorticity shell spectra (fg2_spectrum) of the hydro KH
     state on n x n: pack u1 + i u2, one FP32 complex 2-D FFT, FP64 shell sums;
@@ -445,10 +479,13 @@ def main():
     ap.add_argument("--no-disk", action="store_true")
     ap.add_argument("--no-spectrum", action="store_true")
     ap.add_argument("--no-slab", action="store_true")
+    ap.add_argument("--no-bounded", action="store_true")
     ap.add_argument("--disk-only", action="store_true",
                     help="run only the disk leg and print its JSON (profiling)")
     ap.add_argument("--slab-only", action="store_true",
                     help="run only the slab leg and print its JSON (profiling)")
+    ap.add_argument("--bounded-only", action="store_true",
+                    help="run only the bounded-grid leg and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -461,6 +498,10 @@ def main():
     ev = HipEvents()
     if args.disk_only:
         print(json.dumps({"disk_2048": bench_disk(ev, steps=args.steps, warmup=args.warmup)}),
+              flush=True)
+        return
+    if args.bounded_only:
+        print(json.dumps({"bounded": bench_bounded(ev, steps=args.steps, warmup=args.warmup)}),
               flush=True)
         return
     x = ics.hydro_kh(N, N, seed=2)
@@ -576,6 +617,7 @@ def main():
         "disk_2048": None,
         "spectrum": None,
         "slabs": None,
+        "bounded": None,
         "cpu_baseline": None,
     }
     run.close()
@@ -587,6 +629,8 @@ def main():
         out["disk_2048"] = bench_disk(ev)
     if not args.no_spectrum:
         out["spectrum"] = bench_spectrum()
+    if not args.no_bounded:
+        out["bounded"] = bench_bounded(ev)
     if not args.no_slab:
         out["slabs"] = bench_slab(ev, dt)
         out["slabs"]["share"]["vs_whole_grid_share"] = \

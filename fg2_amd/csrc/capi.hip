@@ -278,6 +278,21 @@ FusedShape step_shape_of(const fg2_run *r)
     const int kind = (r->p1 && r->p2) ? 1 : 4;
     const long slots = (long)blocks_per_cu(r->module, kind, sh.waves) * r->cus;
     sh.tile_h = tile_rows(r->n1, nstrips, slots, 12, 6);
+    // A grid that fills under half of one round is latency bound: a tile
+    // takes tile_h + 12 dependent iterations whatever the load.  Such grids
+    // take one-wavefront workgroups (46 output columns, no waits on other
+    // wavefronts) and the shortest tiles for which the launch keeps at most
+    // 1.5 wavefronts per SIMD.  Measured on hydro 128^2 .. 512^2 this runs
+    // a step in 13-21 us against 32-34 us for 12-row tiles of 4 wavefronts.
+    const long tiles = nstrips * ((r->n1 + sh.tile_h - 1) / sh.tile_h);
+    if(2 * tiles <= slots) {
+      const long strips1 = (r->n2 + 63 - 6 * HALF) / (64 - 6 * HALF);
+      const long cap = 6L * r->cus;                 // 1.5 wavefronts x 4 SIMDs per CU
+      long h = 1;
+      while(h < r->n1 && strips1 * ((r->n1 + h - 1) / h) > cap) ++h;
+      sh.waves = 1;
+      sh.tile_h = (int)h;
+    }
   }
   return sh;
 }

@@ -40,6 +40,18 @@ Pinning (which test fixes each function against something other than itself):
   CFL time step (new; src/getdt.cpp:26-28 is a TODO)
       stability constants: |R(i sqrt3)| = 1, |R(-2.5127)| = 1, max of the
       D1 modified wavenumber                              tests/test_oracle_step.py
+  SplitMix64 (the seeded stream in place of libc rand())
+      the published reference outputs of seed 1234567     tests/test_oracle_ics.py
+  initial conditions (src/init.cpp:26-42, src/*/init.h)
+      KH: the two-layer density, pressure and shear, the noise's bound;
+      blast, Sod, zeros: the piecewise states (hot-cell count by brute
+      force); bow, implos: their limits away from and at the centre; Kf:
+      Box-Muller velocities from the stream's draws (radius and angle);
+      Hawley torus: the polytrope, the Bernoulli constant and the constant
+      specific angular momentum inside the torus      tests/test_oracle_ics.py
+
+The spectral diagnostics have their own restatement and pins in
+oracle/diagnostics.py.
 """
 import ctypes
 import os

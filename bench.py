@@ -103,6 +103,7 @@ def cpu_baseline(budget_s=12.0):
             break
     rate = N * N * steps / el
     return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
             "sample": f"hydro KH {N}x{N} FP32 (the oracle's FP32 instantiation, OpenMP, "
                       f"{steps} timed RK3 steps after one warm-up step) on the host "
                       f"({el:.1f} s, OMP_NUM_THREADS={threads})"}

@@ -1226,7 +1226,8 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
   // the same number of tiles of equal height (at most h rows), which keeps
   // short last tiles from idling their slots for most of the launch.
   std::vector<FusedShape> cand;
-  const int hs[] = {16, 32, 48, 64, 96, 128};
+  // short tiles serve batches that fill under one round (latency bound)
+  const int hs[] = {1, 2, 4, 8, 16, 32, 48, 64, 96, 128};
   for(int w : {4, 2, 1}) {
     for(int h : hs) cand.push_back({h, w});
     // and the two heights the rounds model ranks first for this shape: the
@@ -1235,7 +1236,7 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
     const long slots = (long)batch_grid(b->module, b->step, w, b->cus) * (b->step ? 1 : w);
     const long extra = b->step ? 6 : 3;
     long c1 = -1, c2 = -1, h1 = 0, h2 = 0;
-    for(long h = 8; h <= 256; ++h) {
+    for(long h = 1; h <= 256; ++h) {
       long tiles = 0;
       for(fg2_run *r : b->runs)
         tiles += (long)((r->n2 + out - 1) / out) * ((r->n1 + h - 1) / h);

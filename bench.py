@@ -2,7 +2,8 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--no-batch] [--no-serve] [--no-disk] [--no-spectrum] [--no-slab]
-                    [--no-bounded] [--disk-only] [--slab-only] [--bounded-only]
+                    [--no-bounded] [--no-config1] [--disk-only] [--slab-only]
+                    [--bounded-only]
 
 Workload (the `value`): hydro module, Kelvin-Helmholtz shear layer on a
 3584 x 3584 periodic grid, FP32, one launch of the one-pass step kernel per
@@ -513,6 +514,7 @@ def main():
     ap.add_argument("--no-spectrum", action="store_true")
     ap.add_argument("--no-slab", action="store_true")
     ap.add_argument("--no-bounded", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--disk-only", action="store_true",
                     help="run only the disk leg and print its JSON (profiling)")
     ap.add_argument("--slab-only", action="store_true",
@@ -665,7 +667,8 @@ def main():
         out["spectrum"] = bench_spectrum()
     if not args.no_bounded:
         out["bounded"] = bench_bounded(ev)
-    out["config1_hydro_256"] = bench_config1(ev)
+    if not args.no_config1:
+        out["config1_hydro_256"] = bench_config1(ev)
     if not args.no_slab:
         out["slabs"] = bench_slab(ev, dt)
         out["slabs"]["share"]["vs_whole_grid_share"] = \

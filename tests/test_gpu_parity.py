@@ -228,8 +228,9 @@ def test_batch_equals_alone(fg2):
 def test_step_kernel_equals_substages(fg2, module, n1, n2, p1, p2):
     """The one-pass RK3 step (temporally blocked) gives the bits of three
     substage launches, for any tile and workgroup shape, including grids
-    narrower than one workgroup strip (wrapped halos), and with Neumann rows
-    and / or reflecting columns (the ghost fill applied to every stage)."""
+    narrower than one workgroup strip (wrapped halos), tiles of one row (the
+    small-grid shape), and with Neumann rows and / or reflecting columns (the
+    ghost fill applied to every stage)."""
     _say(f"case step kernel {module} {n1}x{n2} p1={p1} p2={p2}")
     x = state(module, n1, n2, seed=21)
     geo = dict(l1=1.0, l2=1.0, p1=p1, p2=p2, c=0.5)
@@ -238,7 +239,7 @@ def test_step_kernel_equals_substages(fg2, module, n1, n2, p1, p2):
     assert ref.run(0.0, dt, 3) == 0
     y = ref.download()
     assert np.all(np.isfinite(y))
-    for tile in [(0, 4), (8, 1), (37, 2), (128, 4), (300, 1)]:
+    for tile in [(0, 4), (1, 1), (3, 4), (8, 1), (37, 2), (128, 4), (300, 1)]:
         r = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED, geo, tile=tile)
         assert r.run(0.0, dt, 3) == 0
         np.testing.assert_array_equal(r.download(), y, err_msg=f"tile {tile}")

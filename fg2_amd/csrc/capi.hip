@@ -272,19 +272,25 @@ FusedShape step_shape_of(const fg2_run *r)
     // of at least 12 rows: below that the recomputed rows dominate (on a
     // 448 x 3584 slab, 14-row tiles fill one round exactly and were the
     // fastest measured, 7% ahead of 16)
+    // (tiles counted as launched: a narrow last strip takes one workgroup
+    // slot per four row chunks)
     sh.waves = 4;
-    const long out = 64 * sh.waves - 6 * HALF;
-    const long nstrips = (r->n2 + out - 1) / out;
     const int kind = (r->p1 && r->p2) ? 1 : 4;
     const long slots = (long)blocks_per_cu(r->module, kind, sh.waves) * r->cus;
-    sh.tile_h = tile_rows(r->n1, nstrips, slots, 12, 6);
+    long best_cost = -1;
+    sh.tile_h = std::max(1, std::min(r->n1, 12));
+    for(int h = 12; h <= std::min(r->n1, 1024); ++h) {
+      const long cost = ((step_launch_tiles(r->n1, r->n2, h, 4, nullptr) + slots - 1) / slots) *
+                        (h + 6);
+      if(best_cost < 0 || cost < best_cost) { best_cost = cost; sh.tile_h = h; }
+    }
     // A grid that fills under half of one round is latency bound: a tile
     // takes tile_h + 12 dependent iterations whatever the load.  Such grids
     // take one-wavefront workgroups (46 output columns, no waits on other
     // wavefronts) and the shortest tiles for which the launch keeps at most
     // 1.5 wavefronts per SIMD.  Measured on hydro 128^2 .. 512^2 this runs
     // a step in 13-21 us against 32-34 us for 12-row tiles of 4 wavefronts.
-    const long tiles = nstrips * ((r->n1 + sh.tile_h - 1) / sh.tile_h);
+    const long tiles = step_launch_tiles(r->n1, r->n2, sh.tile_h, 4, nullptr);
     if(2 * tiles <= slots) {
       const long strips1 = (r->n2 + 63 - 6 * HALF) / (64 - 6 * HALF);
       const long cap = 6L * r->cus;                 // 1.5 wavefronts x 4 SIMDs per CU

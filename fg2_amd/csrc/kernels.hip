@@ -22,6 +22,7 @@
 // wavefront scope: no s_barrier and no vmcnt drain inside the march.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <climits>
 #include "launch.h"
 #include "physics.h"
 
@@ -315,7 +316,10 @@ template<class M, int NW> struct StepLds {
 };
 
 // One tile of the one-pass step: output rows [i0, i1), output columns
-// [j0, j0 + 64 NW - 18).  All threads of the workgroup call it together.
+// [j0, j0 + WL - 18) on the WL = 64 NW lanes of the workgroup, or (a packed
+// narrow strip: lane0 = 64 w, WL = 64) on wavefront w's own 64 lanes.  All
+// threads of the workgroup call it together with the same nit >= i1 - i0 + 12
+// (iterations past a wavefront's own rows store nothing).
 // STEP_SLAB: the run is rows [H.row0, H.row0 + n1) of a grid of H.n1g rows;
 // rows above and below it come from the halo buffers H.lo and H.hi instead
 // of wrapping, and the right-hand side sees the global row index.
@@ -325,50 +329,52 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
                                           const long s, const Params &P, const float g0,
                                           const float g1, const float g2, const float b1,
                                           const float b2, const int j0, const int i0,
-                                          const int i1, const int cfl, StepLds<M, NW> &L,
+                                          const int i1, const int lane0, const int WL,
+                                          const int nit, const int cfl, StepLds<M, NW> &L,
                                           TileOut &res, const SlabHalo &H = SlabHalo{})
 {
-  constexpr int NV = M::NV, ND = M::ND, TW = 64 * NW, H3 = 3 * HALF;
+  constexpr int NV = M::NV, ND = M::ND, H3 = 3 * HALF;
   auto &X = L.X;
   auto &G = L.G;
   constexpr bool SLAB = MODE == STEP_SLAB, BOUNDED = MODE == STEP_BOUNDED;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, tl = t - lane0;          // LDS lane, lane in the strip
   const int n1 = P.n1, n2 = P.n2;
-  const int c = j0 - H3 + t;
+  const int c = j0 - H3 + tl;
   const bool mirror_cols = BOUNDED && !P.p2, mirror_rows = BOUNDED && !P.p1;
   // this lane's column (lanes outside a reflecting grid load a mirrored
   // column; no kept value reads them)
   const int cc = mirror_cols ? mirror(c, n2) : pmod(c, n2);
-  const bool out = t >= H3 && t < TW - H3 && c < n2;  // lanes of stage-2 output
+  const bool out = tl >= H3 && tl < WL - H3 && c < n2;  // lanes of stage-2 output
   const float *col = uin + (long)cc * NV;
   const float k1a = 1.5859783962656393f / P.Delta1, k1b = 1.5859783962656393f / P.Delta2;
-  // neighbour indices in LDS rows (clamped: lanes near the workgroup edge
-  // compute garbage that no output depends on); with reflecting columns the
-  // lane of the mirrored column, and the sign of the odd variable there
+  // neighbour indices in LDS rows, inside the strip's lanes [lane0, lane0 +
+  // WL) (clamped: lanes near the strip's edge compute garbage that no output
+  // depends on); with reflecting columns the lane of the mirrored column,
+  // and the sign of the odd variable there
+  const int lo_lane = lane0, hi_lane = lane0 + WL - 1;
   int nb[2][3];
   float sg[2][3];
 #pragma unroll
   for(int k = 1; k <= 3; ++k) {
-    nb[0][k - 1] = min(t + k, TW - 1);
-    nb[1][k - 1] = max(t - k, 0);
+    nb[0][k - 1] = min(t + k, hi_lane);
+    nb[1][k - 1] = max(t - k, lo_lane);
     sg[0][k - 1] = sg[1][k - 1] = 1.0f;
     if(mirror_cols) {
       const int cp = c + k, cm = c - k;
-      nb[0][k - 1] = min(max(mirror(cp, n2) - (j0 - H3), 0), TW - 1);
-      nb[1][k - 1] = min(max(mirror(cm, n2) - (j0 - H3), 0), TW - 1);
+      nb[0][k - 1] = min(max(mirror(cp, n2) - (j0 - H3) + lane0, lo_lane), hi_lane);
+      nb[1][k - 1] = min(max(mirror(cm, n2) - (j0 - H3) + lane0, lo_lane), hi_lane);
       sg[0][k - 1] = (cp < 0 || cp >= n2) ? -1.0f : 1.0f;
       sg[1][k - 1] = (cm < 0 || cm >= n2) ? -1.0f : 1.0f;
     }
   }
 
-  const int nit = (i1 - i0) + 4 * HALF;
   // The right-most strip of a grid that the strips do not divide needs only
-  // lanes t <= n2 - j0 + 17 (its last output column plus the 9-column halo).
-  // Wavefronts wholly past that skip the work and keep only the workgroup's
-  // barriers, three per iteration as below.  The lanes next to them read
-  // stale LDS, which only reaches lanes more than 9 columns past the last
-  // output column.
-  if(64 * __builtin_amdgcn_readfirstlane(t >> 6) > n2 - j0 + 2 * H3 - 1) {
+  // lanes tl <= n2 - j0 + 17 (its last output column plus the 9-column halo).
+  // Wavefronts wholly past that, and wavefronts of a packed narrow strip
+  // without rows, skip the work and keep only the workgroup's barriers,
+  // three per iteration as below.  The lanes next to them read stale LDS,
+  // which only reaches lanes more than 9 columns past the last output column.
+  if(64 * __builtin_amdgcn_readfirstlane(t >> 6) - lane0 > n2 - j0 + 2 * H3 - 1 || i1 <= i0) {
     for(int it = 0; it < nit; ++it) { __syncthreads(); __syncthreads(); __syncthreads(); }
     return;
   }
@@ -466,7 +472,7 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
         const int r2 = r0 - 2 * HALF;
         stage(2, [&](int d) -> float (&)[NV] { return W2[(p + 7 + d) & 7]; }, r2,
               V1[(p + 3) & 3], b2, g2, vn, uo);
-        if(out) {
+        if(out && r2 < i1) {
 #pragma unroll
           for(int q = 0; q < NV; ++q) res.bad |= (uo[q] != uo[q]);
           store_cell<NV>(uout + (long)r2 * s + (long)c * NV, uo);
@@ -482,19 +488,51 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
   }
 }
 
+// Tile `tile` of a single-run step launch (step_launch_tiles): nfull strips
+// of 64 NW lanes times the row chunks, then the narrow last strip (at most
+// 46 columns, when the launch packs it) with NW row chunks per workgroup,
+// one to each wavefront, so it holds NW times fewer workgroup slots.
+struct StepTileShape { int j0, i0, i1, lane0, wl, nit; };
+template<int NW>
+__device__ __forceinline__ StepTileShape step_tile_shape(const int tile, const int n1,
+                                                         const int tile_h, const int nfull)
+{
+  constexpr int OUT = 64 * NW - 6 * HALF;
+  const int nchunks = (n1 + tile_h - 1) / tile_h;
+  StepTileShape T;
+  if(tile < nfull * nchunks) {
+    const int strip = tile % nfull, chunk = tile / nfull;
+    T.j0 = strip * OUT;
+    T.i0 = chunk * tile_h;
+    T.i1 = min(n1, T.i0 + tile_h);
+    T.lane0 = 0;
+    T.wl = 64 * NW;
+    T.nit = T.i1 - T.i0 + 4 * HALF;
+  } else {
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int chunk = (tile - nfull * nchunks) * NW + w;
+    T.j0 = nfull * OUT;
+    T.i0 = chunk * tile_h;
+    T.i1 = min(n1, T.i0 + tile_h);          // i1 <= i0: a wavefront without rows
+    T.lane0 = 64 * w;
+    T.wl = 64;
+    T.nit = tile_h + 4 * HALF;              // the same for every wavefront
+  }
+  return T;
+}
+
 template<class M, int NW>
 __global__ __launch_bounds__(64 * NW)
 void step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const long s,
                  const Params P, const float g0, const float g1, const float g2,
-                 const float b1, const float b2, const int tile_h, const int nstrips,
+                 const float b1, const float b2, const int tile_h, const int nfull,
                  const int cfl, DeviceDiag *diag)
 {
   __shared__ StepLds<M, NW> L;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int strip = tile % nstrips, chunk = tile / nstrips;
-  const int i0 = chunk * tile_h, i1 = min(P.n1, i0 + tile_h);
+  const StepTileShape T = step_tile_shape<NW>(xcd_remap(blockIdx.x, gridDim.x), P.n1, tile_h,
+                                              nfull);
   TileOut res{0, 0.0f, 0.0f};
-  step_tile<M, NW>(uin, uout, s, P, g0, g1, g2, b1, b2, strip * (64 * NW - 6 * HALF), i0, i1,
+  step_tile<M, NW>(uin, uout, s, P, g0, g1, g2, b1, b2, T.j0, T.i0, T.i1, T.lane0, T.wl, T.nit,
                    cfl, L, res);
   commit_tile(res, cfl, diag);
 }
@@ -506,16 +544,15 @@ template<class M, int NW>
 __global__ __launch_bounds__(64 * NW)
 void slab_step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const long s,
                       const Params P, const float g0, const float g1, const float g2,
-                      const float b1, const float b2, const int tile_h, const int nstrips,
+                      const float b1, const float b2, const int tile_h, const int nfull,
                       const int cfl, DeviceDiag *diag, const SlabHalo H)
 {
   __shared__ StepLds<M, NW> L;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int strip = tile % nstrips, chunk = tile / nstrips;
-  const int i0 = chunk * tile_h, i1 = min(P.n1, i0 + tile_h);
+  const StepTileShape T = step_tile_shape<NW>(xcd_remap(blockIdx.x, gridDim.x), P.n1, tile_h,
+                                              nfull);
   TileOut res{0, 0.0f, 0.0f};
-  step_tile<M, NW, STEP_SLAB>(uin, uout, s, P, g0, g1, g2, b1, b2, strip * (64 * NW - 6 * HALF),
-                              i0, i1, cfl, L, res, H);
+  step_tile<M, NW, STEP_SLAB>(uin, uout, s, P, g0, g1, g2, b1, b2, T.j0, T.i0, T.i1, T.lane0,
+                              T.wl, T.nit, cfl, L, res, H);
   commit_tile(res, cfl, diag);
 }
 
@@ -526,16 +563,15 @@ template<class M, int NW>
 __global__ __launch_bounds__(64 * NW)
 void bounded_step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const long s,
                          const Params P, const float g0, const float g1, const float g2,
-                         const float b1, const float b2, const int tile_h, const int nstrips,
+                         const float b1, const float b2, const int tile_h, const int nfull,
                          const int cfl, DeviceDiag *diag)
 {
   __shared__ StepLds<M, NW> L;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int strip = tile % nstrips, chunk = tile / nstrips;
-  const int i0 = chunk * tile_h, i1 = min(P.n1, i0 + tile_h);
+  const StepTileShape T = step_tile_shape<NW>(xcd_remap(blockIdx.x, gridDim.x), P.n1, tile_h,
+                                              nfull);
   TileOut res{0, 0.0f, 0.0f};
-  step_tile<M, NW, STEP_BOUNDED>(uin, uout, s, P, g0, g1, g2, b1, b2,
-                                 strip * (64 * NW - 6 * HALF), i0, i1, cfl, L, res);
+  step_tile<M, NW, STEP_BOUNDED>(uin, uout, s, P, g0, g1, g2, b1, b2, T.j0, T.i0, T.i1, T.lane0,
+                                 T.wl, T.nit, cfl, L, res);
   commit_tile(res, cfl, diag);
 }
 
@@ -566,7 +602,8 @@ void batch_step_kernel(const BatchRun *__restrict__ runs, const int nruns, const
     const int i0 = chunk * R.tile_h, i1 = min(R.P.n1, i0 + R.tile_h);
     TileOut res{0, 0.0f, 0.0f};
     step_tile<M, NW>(R.uin, R.uout, R.s, R.P, R.g[0], R.g[1], R.g[2], (float)(-5.0 / 9.0),
-                     (float)(-153.0 / 128.0), strip * (64 * NW - 6 * HALF), i0, i1, 0, L, res);
+                     (float)(-153.0 / 128.0), strip * (64 * NW - 6 * HALF), i0, i1, 0, 64 * NW,
+                     i1 - i0 + 4 * HALF, 0, L, res);
     commit_tile(res, 0, R.diag);
   }
 }
@@ -946,20 +983,22 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
       return hipErrorInvalidValue;   // disk runs take the substage path
     } else {
     const int tile_h = std::max(1, sh.tile_h);
-    const int nchunks = (P.n1 + tile_h - 1) / tile_h;
+    int nfull = 0;
+    const long tiles = step_launch_tiles(P.n1, P.n2, tile_h, sh.waves, &nfull);
+    if(tiles < 1 || tiles > INT_MAX) return hipErrorInvalidValue;
+    const int grid = (int)tiles;
     switch(sh.waves) {
 #define FG2_S(NW)                                                                            \
     case NW: {                                                                               \
-      const int nstrips = (P.n2 + 64 * NW - 6 * HALF - 1) / (64 * NW - 6 * HALF);            \
       if(slab)                                                                               \
-        slab_step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(                      \
-            uin, uout, s, P, g0, g1, g2, b1, b2, tile_h, nstrips, cfl, diag, *slab);           \
+        slab_step_kernel<M, NW><<<grid, 64 * NW, 0, st>>>(                                   \
+            uin, uout, s, P, g0, g1, g2, b1, b2, tile_h, nfull, cfl, diag, *slab);             \
       else if(bounded)                                                                       \
-        bounded_step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(                   \
-            uin, uout, s, P, g0, g1, g2, b1, b2, tile_h, nstrips, cfl, diag);                  \
+        bounded_step_kernel<M, NW><<<grid, 64 * NW, 0, st>>>(                                \
+            uin, uout, s, P, g0, g1, g2, b1, b2, tile_h, nfull, cfl, diag);                    \
       else                                                                                   \
-        step_kernel<M, NW><<<nstrips * nchunks, 64 * NW, 0, st>>>(uin, uout, s, P, g0, g1, g2, \
-                                                              b1, b2, tile_h, nstrips, cfl, diag); \
+        step_kernel<M, NW><<<grid, 64 * NW, 0, st>>>(uin, uout, s, P, g0, g1, g2,            \
+                                                   b1, b2, tile_h, nfull, cfl, diag);          \
       break;                                                                                 \
     }
     FG2_S(1) FG2_S(2) FG2_S(4)

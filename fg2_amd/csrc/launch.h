@@ -39,6 +39,24 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
                        double dt, const FusedShape &shape, int cfl, DeviceDiag *diag,
                        hipStream_t st, const SlabHalo *slab = nullptr);
 
+// Workgroups of a single-run step launch with tiles of tile_h rows and
+// `waves` wavefronts: *nfull strips of 64 waves - 18 output columns (the
+// last one partial when the columns do not divide), each cut into row
+// chunks; and when the columns leave a last strip of at most 46 columns and
+// waves > 1, that narrow strip's chunks packed `waves` to a workgroup, one
+// per wavefront.
+inline long step_launch_tiles(int n1, int n2, int tile_h, int waves, int *nfull)
+{
+  const int out = 64 * waves - 6 * HALF, narrow = 64 - 6 * HALF;
+  const long nchunks = (n1 + tile_h - 1) / tile_h;
+  int full = n2 / out;
+  const int rem = n2 - full * out;
+  const bool pack = waves > 1 && rem > 0 && rem <= narrow;
+  if(rem > 0 && !pack) ++full;
+  if(nfull) *nfull = full;
+  return (long)full * nchunks + (pack ? (nchunks + waves - 1) / waves : 0);
+}
+
 // Fill the disk row table (Params::rowtab, 4 floats per row) of a run
 // from its Delta1 and parameters M, rS, rin.
 hipError_t launch_disk_rows(float *tab, const Params &P, hipStream_t st);

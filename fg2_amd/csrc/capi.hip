@@ -1085,7 +1085,6 @@ __global__ void zero_ints_kernel(int *p, int n)
 
 int batch_fill_step(fg2_batch *b, const double *dts)
 {
-  const int out = 64 * b->waves - 6 * HALF;
   for(int par = 0; par < 2; ++par) {
     auto &H = b->host[0][par];
     H.resize(b->nruns);
@@ -1101,8 +1100,7 @@ int batch_fill_step(fg2_batch *b, const double *dts)
       R.s = r->s;
       R.P = make_params(r);
       R.tile_h = b->tile_h[i];
-      R.nstrips = (r->n2 + out - 1) / out;
-      R.ntiles = R.nstrips * ((r->n1 + R.tile_h - 1) / R.tile_h);
+      R.ntiles = (int)step_launch_tiles(r->n1, r->n2, R.tile_h, b->waves, &R.nstrips);
       R.tile_begin = begin;
       begin += R.ntiles;
       for(int k = 0; k < 3; ++k) R.g[k] = (float)(dts[i] * RK_GAMMA[k]);
@@ -1245,7 +1243,8 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
     for(long h = 1; h <= 256; ++h) {
       long tiles = 0;
       for(fg2_run *r : b->runs)
-        tiles += (long)((r->n2 + out - 1) / out) * ((r->n1 + h - 1) / h);
+        tiles += b->step ? step_launch_tiles(r->n1, r->n2, (int)std::min<long>(h, r->n1), w, nullptr)
+                         : (long)((r->n2 + out - 1) / out) * ((r->n1 + h - 1) / h);
       const long cost = ((tiles + slots - 1) / slots) * (h + extra);
       if(c1 < 0 || cost < c1) { c2 = c1; h2 = h1; c1 = cost; h1 = h; }
       else if(c2 < 0 || cost < c2) { c2 = cost; h2 = h; }
@@ -1545,14 +1544,13 @@ FusedShape server_shape(const fg2_server *S, int m)
   long best_cost = -1;
   for(int w : {4, 2, 1}) {
     if(S->waves > 0 && w != S->waves) continue;
-    const int out = 64 * w - 6 * HALF;
     const long slots = batch_grid(m, true, w, S->cus);
     for(int h : {16, 24, 32, 40, 48, 56, 64, 72, 84, 96, 112, 128}) {
       if(S->tile_h > 0 && h != S->tile_h) continue;
       long tiles = 0;
       for(const Request *q : S->active)
         if(q->module == m && q->done_steps < q->steps)
-          tiles += (long)((q->n2 + out - 1) / out) * ((q->n1 + h - 1) / h);
+          tiles += step_launch_tiles(q->n1, q->n2, std::min(h, q->n1), w, nullptr);
       const long cost = ((tiles + slots - 1) / slots) * (h + 6);
       if(best_cost < 0 || cost < best_cost) { best_cost = cost; best = FusedShape{h, w}; }
     }
@@ -1566,7 +1564,7 @@ int server_launch(fg2_server *S)
     S->rec_host.clear();
     int begin = 0;
     const FusedShape sh = server_shape(S, m);
-    const int th = sh.tile_h, out = 64 * sh.waves - 6 * HALF;
+    const int th = sh.tile_h;
     for(Request *q : S->active) {
       if(q->module != m || q->done_steps >= q->steps) continue;
       BatchRun R{};
@@ -1575,8 +1573,7 @@ int server_launch(fg2_server *S)
       R.s = q->s;
       R.P = q->P;
       R.tile_h = std::min(th, q->n1);
-      R.nstrips = (q->n2 + out - 1) / out;
-      R.ntiles = R.nstrips * ((q->n1 + R.tile_h - 1) / R.tile_h);
+      R.ntiles = (int)step_launch_tiles(q->n1, q->n2, R.tile_h, sh.waves, &R.nstrips);
       R.tile_begin = begin;
       begin += R.ntiles;
       for(int k = 0; k < 3; ++k) R.g[k] = (float)(q->dt * RK_GAMMA[k]);

@@ -597,13 +597,12 @@ void batch_step_kernel(const BatchRun *__restrict__ runs, const int nruns, const
       if(runs[mid].tile_begin <= t) lo = mid; else hi = mid - 1;
     }
     const BatchRun &R = runs[lo];
-    const int lt = t - R.tile_begin;
-    const int strip = lt % R.nstrips, chunk = lt / R.nstrips;
-    const int i0 = chunk * R.tile_h, i1 = min(R.P.n1, i0 + R.tile_h);
+    // the run's tiles as a single-run launch of it cuts them (nstrips strips
+    // of 64 NW lanes, then a packed narrow strip)
+    const StepTileShape T = step_tile_shape<NW>(t - R.tile_begin, R.P.n1, R.tile_h, R.nstrips);
     TileOut res{0, 0.0f, 0.0f};
     step_tile<M, NW>(R.uin, R.uout, R.s, R.P, R.g[0], R.g[1], R.g[2], (float)(-5.0 / 9.0),
-                     (float)(-153.0 / 128.0), strip * (64 * NW - 6 * HALF), i0, i1, 0, 64 * NW,
-                     i1 - i0 + 4 * HALF, 0, L, res);
+                     (float)(-153.0 / 128.0), T.j0, T.i0, T.i1, T.lane0, T.wl, T.nit, 0, L, res);
     commit_tile(res, 0, R.diag);
   }
 }

@@ -245,6 +245,29 @@ def test_step_kernel_equals_substages(fg2, module, n1, n2, p1, p2):
         np.testing.assert_array_equal(r.download(), y, err_msg=f"tile {tile}")
 
 
+@pytest.mark.parametrize("waves", [2, 4])
+def test_batch_narrow_strips_equal_alone(fg2, waves):
+    """Widths that leave a last strip of at most 46 columns: the batched step
+    packs its row chunks one per wavefront (as a single-run launch does);
+    every run still gives the bits it gives alone."""
+    sizes = [(64, 130), (96, 150), (40, 46), (70, 256), (50, 260), (33, 477)]
+    _say(f"case batch narrow strips waves={waves}")
+    xs = [state("hydro", a, b, seed=40 + k) for k, (a, b) in enumerate(sizes)]
+    alone = []
+    for (a, b), x in zip(sizes, xs):
+        r = _run(fg2, "hydro", a, b, x, fg2.PATH_FUSED)
+        dt = 0.2 * r.getdt()
+        assert r.run(0.0, dt, 3) == 0
+        alone.append(r.download())
+    runs = [_run(fg2, "hydro", a, b, x, fg2.PATH_FUSED) for (a, b), x in zip(sizes, xs)]
+    batch = fg2.Batch(runs, waves=waves, tile_h=16)
+    dts = [0.2 * r.getdt() for r in runs]
+    for _ in range(3):
+        assert batch.step(dts) == 0
+    for r, y in zip(runs, alone):
+        np.testing.assert_array_equal(r.download(), y)
+
+
 def test_autotune_keeps_state_and_bits(fg2):
     """The autotuner times kernels on the run's state without changing it,
     and the tuned shape gives the same bits as the automatic one."""

@@ -80,7 +80,8 @@ struct SlabHalo {
 };
 
 // Diagnostics written by the kernels.  flags bit 0: a NaN was produced
-// (the sticky contain_nan of src/drift.cu:21,30); bit 1: an Inf was produced.
+// (the sticky contain_nan of src/drift.cu:21,30).  An Inf is not flagged: as
+// in the reference, only a NaN fails a step (src/step.cpp:38-40).
 struct DeviceDiag {
   unsigned int flags;
   unsigned int pad;

@@ -253,9 +253,11 @@ int fg2_perf_model(const fg2_run *run, double *flops, double *bits, double *alg_
 /* ---- batched execution: many independent runs advanced together ---- */
 
 /* A batch of runs of one module.  Each fg2_batch_step advances every run of
- * the batch by one RK3 step of its own dt (dts[k]) with three persistent
- * launches for the whole batch.  The runs keep their own state; a run
- * gives the same bits alone (fg2_step, fused path) and in a batch. */
+ * the batch by one RK3 step of its own dt (dts[k]) with persistent launches
+ * for the whole batch: one (the one-pass step, when every run is periodic
+ * hydro or hydro-isot) or three (one per substage: disk, or a batch with a
+ * Neumann or reflecting run).  The runs keep their own state; a run gives
+ * the same bits alone (fg2_step, fused path) and in a batch. */
 int fg2_batch_create(fg2_batch **batch, fg2_run *const *runs, int nruns, int waves, int tile_h);
 void fg2_batch_free(fg2_batch *batch);
 int fg2_batch_step_async(fg2_batch *batch, const double *dts);

@@ -173,8 +173,9 @@ int fg2_set_path(fg2_run *run, int path);
 int fg2_set_dt_mode(fg2_run *run, int mode);
 /* Fused-kernel tile: rows per tile and wavefronts per workgroup (1, 2 or
  * 4; the one-pass step kernel gives each workgroup 64 waves - 18 output
- * columns, the substage kernel each wavefront 58).  tile_h = 0 restores the
- * automatic choice (or the autotuned one, see fg2_autotune). */
+ * columns, and a last strip of at most 46 columns one wavefront per row
+ * chunk; the substage kernel gives each wavefront 58).  tile_h = 0 restores
+ * the automatic choice (or the autotuned one, see fg2_autotune). */
 int fg2_set_tile(fg2_run *run, int tile_h, int waves);
 int fg2_get_tile(const fg2_run *run, int *tile_h, int *waves);
 /* Time the run's fused kernel on its own state for candidate tile shapes

@@ -400,6 +400,11 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
   auto stage = [&](const int k, auto &&win, const int i, const float (&vprev)[NV],
                    const float beta, const float gdt, float (&vn)[NV], float (&uo)[NV]) {
     const float (&cen)[NV] = win(0);
+    // Issue priority: wavefronts past the barrier, with the stage's
+    // right-hand side ahead of them, go before wavefronts still publishing
+    // their row (measured 0.6-0.9% on the 3584^2 hydro step; the reverse
+    // order costs 2%)
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for(int q = 0; q < NV; ++q) X[k][q][t] = cen[q];
 #pragma unroll
@@ -408,6 +413,7 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
       for(int d = 0; d < ND; ++d)
         G[k][kk - 1][d][t] = win(kk)[M::D12V[d]] - win(-kk)[M::D12V[d]];
     __syncthreads();
+    __builtin_amdgcn_s_setprio(1);
     typename M::P_t x;
 #pragma unroll
     for(int q = 0; q < NV; ++q) x.c[q] = cen[q];

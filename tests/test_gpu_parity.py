@@ -268,13 +268,17 @@ def test_batch_narrow_strips_equal_alone(fg2, waves):
         np.testing.assert_array_equal(r.download(), y)
 
 
-def test_autotune_keeps_state_and_bits(fg2):
+@pytest.mark.parametrize("per", [1, 0])
+def test_autotune_keeps_state_and_bits(fg2, per):
     """The autotuner times kernels on the run's state without changing it,
-    and the tuned shape gives the same bits as the automatic one."""
+    and the tuned shape gives the same bits as the automatic one (periodic,
+    and Neumann / reflecting on the bounded one-pass kernel)."""
     n1, n2 = 200, 300
+    _say(f"case autotune periodic={per}")
     x = state("hydro", n1, n2, seed=31)
-    a = _run(fg2, "hydro", n1, n2, x, fg2.PATH_FUSED)
-    b = _run(fg2, "hydro", n1, n2, x, fg2.PATH_FUSED)
+    geo = dict(l1=1.0, l2=1.0, p1=per, p2=per, c=0.5)
+    a = _run(fg2, "hydro", n1, n2, x, fg2.PATH_FUSED, geo)
+    b = _run(fg2, "hydro", n1, n2, x, fg2.PATH_FUSED, geo)
     th, w = b.autotune()
     _say("autotuned", th, w)
     np.testing.assert_array_equal(b.download(), x.astype(np.float32))

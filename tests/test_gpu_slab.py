@@ -130,6 +130,23 @@ def test_slabs_full_size_share(fg2):
     np.testing.assert_array_equal(_gather(runs), whole.download())
 
 
+def test_slab_autotune_keeps_bits(fg2):
+    """The autotuner times a slab's own kernel (with its halo buffers) and
+    leaves its state alone; the tuned slabs still step to the whole grid's
+    bits."""
+    n1, n2 = 192, 300
+    _say(f"case slab autotune {n1}x{n2}")
+    x = state("hydro", n1, n2, seed=8)
+    whole = fg2.Run("hydro", n1, n2)
+    whole.upload(x)
+    runs = _slabs(fg2, "hydro", x, [96, 96])
+    fg2.slab_exchange(runs)
+    for r in runs:
+        r.autotune()
+    np.testing.assert_array_equal(_gather(runs), x.astype(np.float32))
+    _step_both(fg2, whole, runs, whole.getdt(), 3)
+
+
 def test_slab_refusals(fg2):
     """Host-side checks: what a slab cannot be, and the whole-grid entry
     points a slab refuses."""

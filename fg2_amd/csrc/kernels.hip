@@ -249,8 +249,8 @@ void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, fl
 }
 
 // ---------------------------------------------------------------------------
-// Whole RK3 step in one pass (periodic runs): temporal blocking of the three
-// substages.  A workgroup of NW wavefronts owns 64 NW consecutive columns
+// Whole RK3 step in one pass (hydro, hydro-isot): temporal blocking of the
+// three substages.  A workgroup of NW wavefronts owns 64 NW consecutive columns
 // (3 x 3 halo columns each side are recomputed, so 64 NW - 18 are output)
 // and marches down rows [i0, i1).  At iteration it the workgroup runs
 //   stage 0 on row r0 = i0 - 6 + it   (input u, from HBM, ring W0),
@@ -262,7 +262,9 @@ void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, fl
 // columns outside the grid wrap (periodic), and the right-hand side sees the
 // wrapped indices, so every halo value equals the value the substage path
 // computes at the wrapped cell, bit for bit.  Horizontal neighbours of a
-// stage's centre row come from LDS (one s_barrier per stage).
+// stage's centre row come from LDS (one s_barrier per stage).  A last strip
+// of at most 46 columns is packed: each wavefront of its workgroups marches
+// its own row chunk on its own 64 lanes (step_tile_shape).
 //
 // Bounded runs (Neumann rows, p1 = 0, and / or reflecting columns, p2 = 0)
 // take the same schedule with the reference's ghost fill applied to every

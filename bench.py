@@ -2,22 +2,29 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--no-batch] [--no-serve] [--no-disk] [--no-spectrum] [--no-slab]
-                    [--no-bounded] [--no-config1] [--disk-only] [--slab-only]
-                    [--bounded-only]
+                    [--no-bounded] [--no-config1] [--no-sweep] [--disk-only]
+                    [--slab-only] [--bounded-only] [--headline-only]
 
 Workload (the `value`): hydro module, Kelvin-Helmholtz shear layer on a
 3584 x 3584 periodic grid, FP32, one launch of the one-pass step kernel per
 RK3 step (the three substages temporally blocked), fixed dt from the CFL
 limit of the initial state.  Inputs are resident in HBM before the timed
 region.  Prints one JSON line with the whole-job cell-updates/s, the
-roofline of the step kernel, and beside it: the substage path on the same
-workload, 64 batched hydro-isot runs, a served request stream, the disk
-module on 2048 x 2048 (Neumann / reflecting boundaries, three substage
-launches per step), the energy and vorticity spectra of the 3584 x 3584
-state, the KH grid with Neumann rows and reflecting columns, the same grid
-cut into 8 slabs of 448 rows (one GPU's share, with its halo exchange), and
-the host CPU baseline (the oracle's FP32
-instantiation on all host cores, on a bounded sample of the same workload).
+roofline of the step kernel, and beside it: a served request stream (and
+the server's capacity, batched against per-run launches), 64 batched
+hydro-isot runs, the disk module on 2048 x 2048 (Neumann / reflecting
+boundaries, three substage launches per step), the energy and vorticity
+spectra of the 3584 x 3584 state, the KH grid with Neumann rows and
+reflecting columns, hydro 256^2, a hydro-isot resolution sweep at the
+reference's benchmark sizes, the KH grid cut into 8 slabs of 448 rows (one
+GPU's share, with its halo exchange), the substage path on the headline
+workload, and the host CPU baseline (the oracle's FP32 instantiation on all
+host cores, on a bounded sample of the same workload).
+
+The other GPU legs run before the headline: in a fresh process the step
+takes about 275 us for its first ~50 launches and reaches its steady 236 us
+only after ~100 launches of sustained load (the GPU's clocks ramp up), so a
+headline of a few steps timed first would measure the ramp.
 """
 import argparse
 import ctypes
@@ -110,25 +117,17 @@ def cpu_baseline(budget_s=12.0):
                       f"({el:.1f} s, OMP_NUM_THREADS={threads})"}
 
 
-def log_uniform_sizes(rng, count, lo, hi, mult):
-    """Grid sides drawn log-uniform on [lo, hi], rounded to a multiple of mult."""
-    v = np.exp(rng.uniform(np.log(lo), np.log(hi), count))
-    return np.clip((np.round(v / mult) * mult).astype(int), lo, hi)
-
-
 def bench_batched(ev, steps=500, nruns=64, seed=7):
     """64 independent hydro-isot runs (c_s = 1, box 2 pi) with sides
     log-uniform in [128, 1024], multiples of 32, seeded Gaussian density
-    perturbations (delta rho / rho = 0.05, |k| <= 8), `steps` RK3 steps each.
+    perturbations (delta rho / rho = 0.05, |k| <= 8), `steps` RK3 steps each
+    (fg2_amd.ics.batched_workload, the workload the GPU tests check).
     Batched: one persistent launch per step for all runs (tile shape from the
     autotuner), 50 steps captured in a HIP graph once and replayed.  Baseline: each run launching its own
     single-run kernel, issued round-robin on the runs' own streams."""
     import fg2_amd
     from fg2_amd import ics
-    rng = np.random.default_rng(seed)
-    n1s = log_uniform_sizes(rng, nruns, 128, 1024, 32)
-    n2s = log_uniform_sizes(rng, nruns, 128, 1024, 32)
-    xs = [ics.isot_density(a, b, seed=seed + 1 + k) for k, (a, b) in enumerate(zip(n1s, n2s))]
+    n1s, n2s, xs = ics.batched_workload(nruns, seed)
     cells = int(np.sum(n1s.astype(np.int64) * n2s))
 
     runs = [fg2_amd.Run("hydro-isot", int(a), int(b)) for a, b in zip(n1s, n2s)]
@@ -203,29 +202,17 @@ def bench_batched(ev, steps=500, nruns=64, seed=7):
                      f"region and replayed {steps // gsteps} times"}
 
 
-def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_h=0):
-    """Poisson stream of run requests (rate per second) mixing hydro and
-    hydro-isot, sides log-uniform in 64..2048, steps uniform in 50..2000,
-    seeded random-field initial states; continuous batching under an HBM
-    budget.  Reports p50 / p99 completion latency and aggregate
-    cell-updates/s."""
+def _serve(arrivals, mods, n1s, n2s, steps, states, budget_gib, waves=0, tile_h=0,
+           mode=0):
+    """Drive a server through a request stream: requests are submitted when
+    the wall clock passes their arrival time (all at once for arrivals of
+    0), the server ticks as fast as the host allows.  Returns the latency
+    array, the status count of failed requests, cell-steps, wall time, ticks
+    and host time inside tick()."""
     import fg2_amd
-    from fg2_amd import ics
-    rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    mods = rng.integers(0, 2, nreq)
-    n1s = log_uniform_sizes(rng, nreq, 64, 2048, 1)
-    n2s = log_uniform_sizes(rng, nreq, 64, 2048, 1)
-    steps = rng.integers(50, 2001, nreq)
-    # initial states are generated before the timed loop (host set-up)
-    states = []
-    for k in range(nreq):
-        a, b = int(n1s[k]), int(n2s[k])
-        if mods[k] == 0:
-            states.append(ics.hydro_random(a, b, seed=1000 + k, dtype=np.float32))
-        else:
-            states.append(ics.isot_density(a, b, seed=1000 + k, dtype=np.float32))
+    nreq = len(arrivals)
     srv = fg2_amd.Server(int(budget_gib * 2 ** 30), waves=waves, tile_h=tile_h)
+    srv.set_mode(mode)
     ids = []
     k = 0
     ticks, tick_s = 0, 0.0
@@ -233,9 +220,7 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_
     while True:
         now = time.perf_counter() - t0
         while k < nreq and arrivals[k] <= now:
-            m = "hydro" if mods[k] == 0 else "hydro-isot"
-            ids.append(srv.submit(m, states[k], int(steps[k]), arrivals[k]))
-            states[k] = None
+            ids.append(srv.submit(mods[k], states[k], int(steps[k]), arrivals[k]))
             k += 1
         th0 = time.perf_counter()
         act, que, fin = srv.tick(th0 - t0)
@@ -253,16 +238,59 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_
         bad += st != 0
     cell_steps, _ = srv.stats()
     srv.close()
-    lat = np.array(lat)
-    return {"workload": f"{nreq} requests, Poisson {rate:g}/s, hydro + hydro-isot, "
-                        "sides log-uniform 64..2048, steps U[50,2000], FP32",
-            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "mean_s": float(lat.mean()), "cell_updates_per_s": cell_steps / wall,
-            "wall_s": wall, "ticks": ticks, "host_s_in_tick": tick_s,
-            "hbm_budget_gib": budget_gib,
-            "waves": waves if waves > 0 else "chosen per launch (rounds of resident workgroups)",
-            "tile_h": tile_h if tile_h > 0 else "chosen per launch (rounds of resident workgroups)",
-            "nan_requests": int(bad)}
+    return np.array(lat), int(bad), cell_steps, wall, ticks, tick_s
+
+
+def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_h=0,
+                  ncap=300):
+    """Poisson stream of run requests (rate per second) mixing hydro and
+    hydro-isot, sides log-uniform in 64..2048, steps uniform in 50..2000,
+    seeded random-field initial states (fg2_amd.ics.serving_workload);
+    continuous batching under an HBM budget.  Reports p50 / p99 completion
+    latency and aggregate cell-updates/s of the Poisson stream (whose rate is
+    the offered load), then the server's capacity: the first `ncap` requests
+    all arriving at once (the queue never drains until the end), served
+    batched and served per run (each run's own one-pass launch per step on
+    8 streams, same admission and arena), and their ratio."""
+    from fg2_amd import ics
+    arrivals, mods, n1s, n2s, steps, make = ics.serving_workload(nreq, rate, seed)
+    # initial states are generated before the timed loop (host set-up)
+    states = [make(k) for k in range(nreq)]
+    lat, bad, cell_steps, wall, ticks, tick_s = _serve(arrivals, mods, n1s, n2s, steps, states,
+                                                      budget_gib, waves, tile_h)
+    out = {"workload": f"{nreq} requests, Poisson {rate:g}/s, hydro + hydro-isot, "
+                       "sides log-uniform 64..2048, steps U[50,2000], FP32",
+           "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+           "mean_s": float(lat.mean()), "cell_updates_per_s": cell_steps / wall,
+           "note": "cell_updates_per_s of the Poisson stream is the offered load (the "
+                   "arrivals span about nreq / rate seconds); capacity is below",
+           "wall_s": wall, "ticks": ticks, "host_s_in_tick": tick_s,
+           "host_frac_of_wall": tick_s / wall,
+           "hbm_budget_gib": budget_gib,
+           "waves": waves if waves > 0 else "chosen per launch (rounds of resident workgroups)",
+           "tile_h": tile_h if tile_h > 0 else "chosen per launch (rounds of resident workgroups)",
+           "nan_requests": bad}
+    zero = np.zeros(ncap)
+    cap = {}
+    for name, mode in (("batched", 0), ("per_run", 1)):
+        lat, bad, cell_steps, wall, ticks, tick_s = _serve(zero, mods[:ncap], n1s[:ncap],
+                                                          n2s[:ncap], steps[:ncap],
+                                                          states[:ncap], budget_gib, waves,
+                                                          tile_h, mode)
+        cap[name] = {"cell_updates_per_s": cell_steps / wall, "wall_s": wall, "ticks": ticks,
+                     "host_s_in_tick": tick_s, "p50_s": float(np.percentile(lat, 50)),
+                     "p99_s": float(np.percentile(lat, 99)), "nan_requests": bad}
+    out["capacity_cell_updates_per_s"] = cap["batched"]["cell_updates_per_s"]
+    out["capacity"] = {"workload": f"the first {ncap} requests of the stream, all arriving "
+                                   "at t = 0 (a saturated queue)",
+                       "batched": cap["batched"], "per_run": cap["per_run"],
+                       "speedup_vs_per_run": cap["batched"]["cell_updates_per_s"] /
+                                             cap["per_run"]["cell_updates_per_s"],
+                       "per_run_note": "each admitted run launches its own one-pass step "
+                                       "kernel (the shape fg2_step gives a run of its "
+                                       "size) per tick, on 8 streams; same admission, "
+                                       "arena and copies"}
+    return out
 
 
 def bench_disk(ev, n=2048, steps=200, warmup=10):
@@ -500,6 +528,79 @@ def cpu_config1(steps=100):
             **out}
 
 
+# The reference's published hydro-isot sweep on a Tesla M2090 (best block
+# shape 4 x 128, bench/bench.txt:57-73): side -> ms per RK3 step.
+REF_ISOT_M2090_MS = {1024: 5.468, 2048: 21.188, 3584: 71.384, 4096: 84.107, 8192: 337.530,
+                     14336: 1036.88}
+
+
+def bench_sweep(ev, sides=(128, 256, 512, 1024, 2048, 3584, 4096, 8192, 14336), steps=40,
+                warmup=20):
+    """hydro-isot on n x n periodic grids from the device initial condition
+    Kf (the module's own IC, SplitMix64 noise), the automatic launch shape,
+    closed-form dt: ms per RK3 step at each side, the reference's M2090
+    figure where it publishes one, and the one-pass kernel's tile."""
+    import fg2_amd
+    out = []
+    for n in sides:
+        run = fg2_amd.Run("hydro-isot", n, n)
+        run.init("Kf", 5)
+        dt = run.getdt()
+        st = run.stream()
+        for k in range(warmup):
+            run.step_async(k * dt, dt)
+        ev.sync(st)
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0, st)
+        for k in range(steps):
+            run.step_async(k * dt, dt)
+        ev.record(e1, st)
+        ms = ev.elapsed_ms(e0, e1) / steps
+        tile_h, waves = run.tile()
+        row = {"n": n, "ms_per_step": ms, "cell_updates_per_s": n * n / (ms * 1e-3),
+               "status": run.check(), "tile": {"tile_h": tile_h, "waves": waves}}
+        if n in REF_ISOT_M2090_MS:
+            row["reference_m2090_ms_per_step"] = REF_ISOT_M2090_MS[n]
+            row["vs_reference_m2090"] = REF_ISOT_M2090_MS[n] / ms
+        out.append(row)
+        run.close()
+    return {"workload": "hydro-isot n x n periodic, device IC Kf, closed-form dt, "
+                        f"{steps} timed RK3 steps after {warmup}",
+            "rows": out}
+
+
+def headline(ev, run, dt, steps, warmup):
+    """The timed region of the headline: W untimed steps, then exactly K
+    steps between two events on the run's stream (wall clock around them
+    too); then 25 more launches with an event after each, for the median
+    launch time."""
+    stream = run.stream()
+    for k in range(warmup):
+        run.step_async(k * dt, dt)
+    assert run.check() == 0
+    sums0 = run.sums()                                  # FP64 mass and momentum
+    e0, e1 = ev.event(), ev.event()
+    ev.sync(stream)
+    t0 = time.perf_counter()
+    ev.record(e0, stream)
+    for k in range(steps):
+        run.step_async(k * dt, dt)
+    ev.record(e1, stream)
+    ev.sync(stream)
+    wall = time.perf_counter() - t0
+    gpu_ms = ev.elapsed_ms(e0, e1)
+    status = run.check()
+    sums1 = run.sums()
+    evs = [ev.event() for _ in range(26)]
+    ev.record(evs[0], stream)
+    for k in range(25):
+        run.step_async(k * dt, dt)
+        ev.record(evs[k + 1], stream)
+    series = [1e3 * ev.elapsed_ms(evs[k], evs[k + 1]) for k in range(25)]
+    status = max(status, run.check())
+    return wall, gpu_ms, status, sums0, sums1, series
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -515,12 +616,15 @@ def main():
     ap.add_argument("--no-slab", action="store_true")
     ap.add_argument("--no-bounded", action="store_true")
     ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--disk-only", action="store_true",
                     help="run only the disk leg and print its JSON (profiling)")
     ap.add_argument("--slab-only", action="store_true",
                     help="run only the slab leg and print its JSON (profiling)")
     ap.add_argument("--bounded-only", action="store_true",
                     help="run only the bounded-grid leg and print its JSON (profiling)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline step and its JSON line (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -549,26 +653,34 @@ def main():
         print(json.dumps({"slabs": bench_slab(ev, dt, steps=args.steps, warmup=args.warmup)}),
               flush=True)
         return
-    stream = run.stream()
-    for k in range(args.warmup):
-        run.step_async(k * dt, dt)
-    assert run.check() == 0
-    sums0 = run.sums()                                  # FP64 mass and momentum
-    e0, e1 = ev.event(), ev.event()
-    ev.sync(stream)
-    t0 = time.perf_counter()
-    ev.record(e0, stream)
-    for k in range(args.steps):
-        run.step_async(k * dt, dt)
-    ev.record(e1, stream)
-    ev.sync(stream)
-    wall = time.perf_counter() - t0
-    gpu_ms = ev.elapsed_ms(e0, e1)
-    status = run.check()
+    if args.headline_only:
+        for flag in ("no_batch", "no_serve", "no_disk", "no_spectrum", "no_slab", "no_bounded",
+                     "no_config1", "no_sweep", "no_cpu"):
+            setattr(args, flag, True)
+    # the other GPU legs first (see the module docstring), the GPU-heavy
+    # slab leg last before the headline
+    legs = {}
+    if not args.no_serve:
+        legs["serving"] = bench_serving()
+    if not args.no_batch:
+        legs["batched"] = bench_batched(ev)
+    if not args.no_disk:
+        legs["disk_2048"] = bench_disk(ev)
+    if not args.no_spectrum:
+        legs["spectrum"] = bench_spectrum()
+    if not args.no_config1:
+        legs["config1_hydro_256"] = bench_config1(ev)
+    if not args.no_sweep:
+        legs["sweep_hydro_isot"] = bench_sweep(ev)
+    if not args.no_bounded:
+        legs["bounded"] = bench_bounded(ev)
+    if not args.no_slab:
+        legs["slabs"] = bench_slab(ev, dt)
+
+    wall, gpu_ms, status, sums0, sums1, series = headline(ev, run, dt, args.steps, args.warmup)
     if status != 0:
         print("NaN in the benchmark state", file=sys.stderr)
         sys.exit(1)
-    sums1 = run.sums()
     conservation = {"steps": args.steps,
                     "mass_rel_drift": float(sums1[0] / sums0[0] - 1.0),
                     "momentum1_drift_per_mass": float((sums1[1] - sums0[1]) / sums0[0]),
@@ -584,7 +696,6 @@ def main():
     alg_step = 10 * nv * 4 * cells                      # bytes per RK3 step
     # periodic hydro runs take one launch of the one-pass step kernel per step
     launch_s = gpu_ms * 1e-3 / args.steps
-    achieved = alg_step / launch_s
     traffic, valu_insts = load_counters()
     tile_h, waves = run.tile()
     pm = run.perf_model()
@@ -595,8 +706,43 @@ def main():
                  "gbytes_per_s": pm["bits"] / 8 / (launch_s * 1e9),
                  "note": "eqns_flop + 2 NVAR flop per cell and substage, and 18 NVAR "
                          "R-words per cell per step plus ghosts, as the reference counts"}
+    roofline = {
+        "bound": "valu",
+        "achieved": None if valu_insts is None else valu_insts / launch_s / 1e9,
+        "peak": VALU_PEAK / 1e9, "unit": "G wave-instructions/s",
+        "frac": None if valu_insts is None else valu_insts / launch_s / VALU_PEAK,
+        "traffic": traffic,
+        "kernel": "step_kernel<Hydro,4>: one launch per RK3 step, the three substages "
+                  "temporally blocked",
+        "insts_per_launch": valu_insts,
+        "avg_launch_us": launch_s * 1e6,
+        "median_launch_us": float(np.median(series)),
+        "launch_series_us": series,
+        "note": "the bound that holds is VALU issue: SQ_INSTS_VALU per launch (rocprofv3 "
+                "counters, profiles/) / the live launch time, against 256 CUs x 4 SIMDs x "
+                "2.4 GHz / 4 cycles per FP32 wave64 instruction (packed FP32 off)",
+        "hbm": None if traffic is None else {
+            "achieved": traffic / launch_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": traffic / launch_s / HBM_PEAK,
+            "note": "measured HBM traffic per launch (FETCH_SIZE + WRITE_SIZE, calibrated "
+                    "on a copy kernel) / the live launch time"},
+        "alg_substage_equivalent": {
+            "bytes_per_launch": alg_step, "achieved": alg_step / launch_s / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg_step / launch_s / HBM_PEAK,
+            "note": "B_alg = 10 NVAR 4 B per cell: the bytes three one-pass substage "
+                    "launches would move; this temporally blocked kernel reads and writes "
+                    "the state once, so this is not an HBM fraction"},
+        "alg_flops": {
+            "flops_per_launch": pm["flops"], "achieved": pm["flops"] / launch_s / 1e12,
+            "peak": 78.6, "unit": "TFLOP/s",
+            "frac": pm["flops"] / launch_s / 78.6e12,
+            "note": "the reference's flop model (eqns_flop + 2 NVAR per cell and substage) "
+                    "against MI355X FP32 vector FMA issue without packed FP32 "
+                    "(157.3 TFLOP/s with it)"},
+    }
 
     # the same workload on the substage path (three launches per step), for reference
+    stream = run.stream()
     run.set_path(fg2_amd.PATH_SUBSTAGE)
     run.set_tile(32, 4)
     for k in range(5):
@@ -627,23 +773,9 @@ def main():
                    "tile_h": tile_h, "waves": waves},
         "ns_per_cell_step": 1e9 * wall / (cells * args.steps),
         "hbm_frac_alg": value * 10 * nv * 4 / HBM_PEAK,
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": traffic,
-                     "kernel": "step_kernel<Hydro,4>: one launch per RK3 step, the three "
-                               "substages temporally blocked",
-                     "alg_bytes_per_launch": alg_step,
-                     "avg_launch_us": launch_s * 1e6,
-                     "note": "achieved = algorithmic bytes (10 NVAR 4 B per cell) / launch "
-                             "time; measured HBM traffic per launch is `traffic`",
-                     "valu": None if valu_insts is None else {
-                         "insts_per_launch": valu_insts,
-                         "achieved": valu_insts / launch_s / 1e9, "peak": VALU_PEAK / 1e9,
-                         "unit": "G wave-instructions/s",
-                         "frac": valu_insts / launch_s / VALU_PEAK,
-                         "note": "the bound that holds: SQ_INSTS_VALU per launch (counters) / "
-                                 "launch time, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles "
-                                 "per FP32 wave64 instruction (packed FP32 off)"}},
+        "hbm_frac_alg_note": "cell-updates/s x B_alg (160 B per hydro cell, substage-"
+                             "equivalent) / 8 TB/s; the measured HBM fraction is roofline.hbm",
+        "roofline": roofline,
         "reference_perf_model": ref_model,
         "substage_path_ms_per_step": substage_ms,
         "conservation": conservation,
@@ -654,25 +786,14 @@ def main():
         "slabs": None,
         "bounded": None,
         "config1_hydro_256": None,
+        "sweep_hydro_isot": None,
         "cpu_baseline": None,
     }
-    run.close()
-    if not args.no_batch:
-        out["batched"] = bench_batched(ev)
-    if not args.no_serve:
-        out["serving"] = bench_serving()
-    if not args.no_disk:
-        out["disk_2048"] = bench_disk(ev)
-    if not args.no_spectrum:
-        out["spectrum"] = bench_spectrum()
-    if not args.no_bounded:
-        out["bounded"] = bench_bounded(ev)
-    if not args.no_config1:
-        out["config1_hydro_256"] = bench_config1(ev)
-    if not args.no_slab:
-        out["slabs"] = bench_slab(ev, dt)
+    out.update(legs)
+    if out["slabs"] is not None:
         out["slabs"]["share"]["vs_whole_grid_share"] = \
             ms_step / 8 / out["slabs"]["share"]["ms_per_step"]
+    run.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_baseline"]["config1_hydro_256"] = cpu_config1()

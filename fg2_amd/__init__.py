@@ -39,6 +39,7 @@ FIELDS = {
 OK, ERR_NAN, ERR_HIP, ERR_SHAPE, ERR_ARG, ERR_NOMEM = range(6)
 PATH_FUSED, PATH_PLAIN, PATH_SUBSTAGE = 0, 1, 2
 DT_CLOSED, DT_CFL = 0, 1
+SERVE_BATCHED, SERVE_PER_RUN = 0, 1
 
 
 class Fg2Error(RuntimeError):
@@ -119,6 +120,7 @@ def _load():
         "fg2_server_tick": (I, [P, D, IP, IP, IP]),
         "fg2_server_result": (I, [P, ctypes.c_longlong, FP, DP, IP]),
         "fg2_server_stats": (I, [P, ctypes.POINTER(ctypes.c_longlong), SP]),
+        "fg2_server_set_mode": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -489,6 +491,11 @@ class Server:
         _check(lib.fg2_server_result(self.h, rid, _fp(out) if fetch else None, _dp(t),
                                      ctypes.byref(st)))
         return st.value, tuple(t.tolist()), (out if st.value >= 0 else None)
+
+    def set_mode(self, mode):
+        """SERVE_BATCHED (one batched launch per module per tick) or
+        SERVE_PER_RUN (each run's own step launch, the baseline)."""
+        _check(lib.fg2_server_set_mode(self.h, int(mode)))
 
     def stats(self):
         c, b = ctypes.c_longlong(), ctypes.c_size_t()

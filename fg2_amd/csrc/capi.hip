@@ -103,6 +103,16 @@ __global__ void reset_diag_kernel(DeviceDiag *d, int lam_only)
   d->lam_diff_bits = 0;
 }
 
+// After timed launches on a run's state: NaN flag back to what it was
+// before the timing (a NaN of an earlier unchecked step stays reported),
+// CFL rates cleared.
+__global__ void restore_diag_kernel(DeviceDiag *d, unsigned int flags)
+{
+  d->flags = flags;
+  d->lam_adv_bits = 0;
+  d->lam_diff_bits = 0;
+}
+
 } // namespace
 
 struct fg2_run {
@@ -796,6 +806,10 @@ extern "C" int fg2_autotune(fg2_run *r, int *tile_h, int *waves)
     if(it != g_tune_cache.end()) best = it->second;
   }
   if(best.tile_h == 0) {
+    // the NaN flag of steps queued before the tuning is kept
+    HIP_TRY(hipMemcpyAsync(r->host_diag, r->diag, sizeof(DeviceDiag), hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    const unsigned int flags = r->host_diag->flags;
     std::vector<FusedShape> cand;
     const FusedShape a = step ? step_shape_of(r) : shape_of(r);
     cand.push_back(a);
@@ -814,7 +828,7 @@ extern "C" int fg2_autotune(fg2_run *r, int *tile_h, int *waves)
       if(int e = time_launch(r, c, step, &ms)) return e;
       if(ms < best_ms) { best_ms = ms; best = c; }
     }
-    reset_diag_kernel<<<1, 1, 0, r->stream>>>(r->diag, 0);
+    restore_diag_kernel<<<1, 1, 0, r->stream>>>(r->diag, flags);
     HIP_TRY(hipStreamSynchronize(r->stream));
     std::lock_guard<std::mutex> lk(g_tune_mutex);
     g_tune_cache[key] = best;
@@ -1113,7 +1127,12 @@ int batch_fill_step(fg2_batch *b, const double *dts)
   return FG2_OK;
 }
 
-int batch_fill(fg2_batch *b, const double *dts)
+// Launch records of one batched step of every run.  timing: records for the
+// autotuner that leave every run's state unchanged -- each substage reads
+// the state u[cur] and writes only the other state buffer and v (which is
+// dead at the start of a step); the one-pass step already writes only the
+// other state buffer.
+int batch_fill(fg2_batch *b, const double *dts, bool timing = false)
 {
   if(b->step) return batch_fill_step(b, dts);
   // stage k reads u[cur ^ (k & 1)] and writes u[cur ^ 1 ^ (k & 1)] of each
@@ -1126,9 +1145,10 @@ int batch_fill(fg2_batch *b, const double *dts)
       for(int i = 0; i < b->nruns; ++i) {
         fg2_run *r = b->runs[i];
         const int cur = r->cur ^ par;
+        const int flip = timing ? 0 : (k & 1);
         BatchRun &R = H[i];
-        R.uin = r->in[cur ^ (k & 1)];
-        R.uout = r->in[cur ^ 1 ^ (k & 1)];
+        R.uin = r->in[cur ^ flip];
+        R.uout = r->in[cur ^ 1 ^ flip];
         R.v = r->in[2];
         R.s = r->s;
         R.P = make_params(r);
@@ -1224,7 +1244,14 @@ void *fg2_batch_stream(fg2_batch *b) { return b ? (void *)b->stream : nullptr; }
 int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
 {
   if(!b || !dts) return fail(FG2_ERR_ARG, "NULL argument");
-  for(fg2_run *r : b->runs) HIP_TRY(hipStreamSynchronize(r->stream));
+  // the NaN flags of steps queued before the tuning are kept
+  std::vector<unsigned int> flags;
+  for(fg2_run *r : b->runs) {
+    HIP_TRY(hipMemcpyAsync(r->host_diag, r->diag, sizeof(DeviceDiag), hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    flags.push_back(r->host_diag->flags);
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
   // A candidate is a tile height h and a width; each is timed twice: every
   // run cut into h-row tiles (the last one shorter), and every run cut into
   // the same number of tiles of equal height (at most h rows), which keeps
@@ -1272,7 +1299,7 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
     b->grid = batch_grid(b->module, b->step, c.waves, b->cus);
     for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = height(c.tile_h, even, i);
     HIP_TRY(hipStreamSynchronize(b->stream));
-    if(int e = batch_fill(b, dts)) return e;
+    if(int e = batch_fill(b, dts, true)) return e;
     if(int e = batch_launch(b, 0)) return e;
     float ms = 1e30f;                       // the least of three timed launches
     for(int rep = 0; rep < 3; ++rep) {
@@ -1291,7 +1318,10 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
   b->waves = best.waves;
   b->grid = batch_grid(b->module, b->step, best.waves, b->cus);
   for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = height(best.tile_h, best_even, i);
-  for(fg2_run *r : b->runs) reset_diag_kernel<<<1, 1, 0, b->stream>>>(r->diag, 0);
+  for(int i = 0; i < b->nruns; ++i) {
+    restore_diag_kernel<<<1, 1, 0, b->stream>>>(b->runs[i]->diag, flags[i]);
+    b->runs[i]->lam_valid = false;          // the CFL rates were cleared
+  }
   HIP_TRY(hipStreamSynchronize(b->stream));
   if(tile_h) *tile_h = best.tile_h;
   if(waves) *waves = best.waves;
@@ -1399,6 +1429,8 @@ struct Request {
   int cur = 0, diag_slot = -1;
   Params P{};
   double dt = 0;
+  FusedShape shape{0, 4};              // per-run mode: the run's own launch shape
+  int sidx = 0;                        // per-run mode: the pool stream it steps on
   int status = FG2_OK;
   bool active = false, retiring = false;
   hipEvent_t copied = nullptr;
@@ -1437,6 +1469,13 @@ struct fg2_server {
   hipEvent_t up_done[STAGES] = {}, down_done[STAGES] = {};
   Request *down_owner[STAGES] = {};
   int up_slot = 0;
+  // per-run mode (FG2_SERVE_PER_RUN): each active run launches its own
+  // one-pass step on one of NPOOL streams; at most SLOTS ticks in flight
+  int mode = FG2_SERVE_BATCHED;
+  static constexpr int NPOOL = 8;
+  hipStream_t pool[NPOOL] = {};
+  hipEvent_t tick_ev[SLOTS][NPOOL] = {};
+  int tick_slot = 0, next_sidx = 0;
 };
 
 namespace {
@@ -1499,6 +1538,12 @@ int server_grow(fg2_server *S, size_t need)
   return FG2_OK;
 }
 
+// The stream a request's steps run on.
+hipStream_t run_stream(const fg2_server *S, const Request *q)
+{
+  return S->mode == FG2_SERVE_PER_RUN ? S->pool[q->sidx] : S->stream;
+}
+
 int server_admit(fg2_server *S, double now)
 {
   while(!S->queue.empty() && !S->free_diag.empty()) {
@@ -1524,7 +1569,11 @@ int server_admit(fg2_server *S, double now)
     HIP_TRY(hipMemcpy2DAsync(q->in[0], q->s * sizeof(float), S->up[u], hp, hp, q->n1,
                              hipMemcpyHostToDevice, S->copy));
     HIP_TRY(hipEventRecord(S->up_done[u], S->copy));
-    HIP_TRY(hipStreamWaitEvent(S->stream, S->up_done[u], 0));  // steps after the upload
+    if(S->mode == FG2_SERVE_PER_RUN) {
+      q->sidx = S->next_sidx;
+      S->next_sidx = (S->next_sidx + 1) % fg2_server::NPOOL;
+    }
+    HIP_TRY(hipStreamWaitEvent(run_stream(S, q), S->up_done[u], 0));  // steps after the upload
     q->t_admit = now;
     q->active = true;
     S->in_use += q->bytes;
@@ -1558,8 +1607,29 @@ FusedShape server_shape(const fg2_server *S, int m)
   return best;
 }
 
+// Per-run mode: every active run's own one-pass step, launched on its pool
+// stream with the shape a single run of its size takes (fg2_step's).
+int server_launch_per_run(fg2_server *S)
+{
+  const int slot = S->tick_slot;
+  S->tick_slot = (S->tick_slot + 1) % fg2_server::SLOTS;
+  for(int k = 0; k < fg2_server::NPOOL; ++k)
+    HIP_TRY(hipEventSynchronize(S->tick_ev[slot][k]));     // at most SLOTS ticks queued
+  for(Request *q : S->active) {
+    if(q->done_steps >= q->steps) continue;
+    HIP_TRY(launch_step(q->module, q->in[q->cur], q->in[q->cur ^ 1], q->s, q->P, q->dt, q->shape,
+                        0, S->diags + q->diag_slot, S->pool[q->sidx]));
+    q->cur ^= 1;
+    ++q->done_steps;
+    S->cell_steps += (long long)q->n1 * q->n2;
+  }
+  for(int k = 0; k < fg2_server::NPOOL; ++k) HIP_TRY(hipEventRecord(S->tick_ev[slot][k], S->pool[k]));
+  return FG2_OK;
+}
+
 int server_launch(fg2_server *S)
 {
+  if(S->mode == FG2_SERVE_PER_RUN) return server_launch_per_run(S);
   for(int m = 0; m < 2; ++m) {
     S->rec_host.clear();
     int begin = 0;
@@ -1613,7 +1683,7 @@ int server_retire(fg2_server *S, double now)
       if(d < 0) break;                                          // retry next tick
       const size_t hp = (size_t)q->n2 * q->nv * sizeof(float);
       if(int e = stage_buffer(&S->down[d], &S->down_cap[d], hp * q->n1)) return e;
-      HIP_TRY(hipEventRecord(S->down_done[d], S->stream));     // the run's last step
+      HIP_TRY(hipEventRecord(S->down_done[d], run_stream(S, q)));  // the run's last step
       HIP_TRY(hipStreamWaitEvent(S->copy, S->down_done[d], 0));
       HIP_TRY(hipMemcpy2DAsync(S->down[d], hp, q->in[q->cur], q->s * sizeof(float), hp, q->n1,
                                hipMemcpyDeviceToHost, S->copy));
@@ -1680,6 +1750,13 @@ int fg2_server_create(fg2_server **out, size_t hbm_budget, int waves, int tile_h
   for(int k = 0; k < fg2_server::SLOTS && e == hipSuccess; ++k)
     e = hipEventCreateWithFlags(&S->rec_done[k], hipEventDisableTiming);
   for(int k = 0; k < fg2_server::SLOTS && e == hipSuccess; ++k) e = hipEventRecord(S->rec_done[k], S->stream);
+  for(int j = 0; j < fg2_server::NPOOL && e == hipSuccess; ++j) {
+    e = hipStreamCreateWithFlags(&S->pool[j], hipStreamNonBlocking);
+    for(int k = 0; k < fg2_server::SLOTS && e == hipSuccess; ++k) {
+      e = hipEventCreateWithFlags(&S->tick_ev[k][j], hipEventDisableTiming);
+      if(e == hipSuccess) e = hipEventRecord(S->tick_ev[k][j], S->pool[j]);
+    }
+  }
   if(e != hipSuccess) {
     const int code = e == hipErrorOutOfMemory ? FG2_ERR_NOMEM : FG2_ERR_HIP;
     fg2_server_free(S);
@@ -1696,6 +1773,11 @@ void fg2_server_free(fg2_server *S)
   if(!S) return;
   if(S->stream) (void)hipStreamSynchronize(S->stream);
   if(S->copy) (void)hipStreamSynchronize(S->copy);
+  for(int j = 0; j < fg2_server::NPOOL; ++j) {
+    if(S->pool[j]) (void)hipStreamSynchronize(S->pool[j]);
+    for(int k = 0; k < fg2_server::SLOTS; ++k) if(S->tick_ev[k][j]) (void)hipEventDestroy(S->tick_ev[k][j]);
+    if(S->pool[j]) (void)hipStreamDestroy(S->pool[j]);
+  }
   for(auto &kv : S->all) delete kv.second;
   for(int k = 0; k < fg2_server::STAGES; ++k) {
     if(S->up[k]) (void)hipHostFree(S->up[k]);
@@ -1734,11 +1816,12 @@ int fg2_server_submit(fg2_server *S, const char *module, int n1, int n2, int ste
   // geometry, parameters and dt of a default run of this module (periodic,
   // src/global.cpp:28-29 and the module's settings()), no device memory
   fg2_run tmp;
-  tmp.module = m; tmp.nv = q->nv; tmp.n1 = n1; tmp.n2 = n2;
+  tmp.module = m; tmp.nv = q->nv; tmp.n1 = n1; tmp.n2 = n2; tmp.cus = S->cus;
   for(auto &p : param_defs(m)) tmp.par.push_back(p.value);
   fg2_config(&tmp);
   q->P = make_params(&tmp);
   q->dt = fg2_getdt(&tmp);
+  q->shape = step_shape_of(&tmp);
   q->t_arrive = now;
   S->queue.push_back(q);
   S->all[q->id] = q;
@@ -1756,6 +1839,7 @@ int fg2_server_tick(fg2_server *S, double now, int *active, int *queued, int *fi
   if(S->active.empty() && S->queue.empty()) {
     HIP_TRY(hipStreamSynchronize(S->stream));
     HIP_TRY(hipStreamSynchronize(S->copy));
+    for(int j = 0; j < fg2_server::NPOOL; ++j) HIP_TRY(hipStreamSynchronize(S->pool[j]));
   }
   if(active) *active = (int)S->active.size();
   if(queued) *queued = (int)S->queue.size();
@@ -1772,6 +1856,15 @@ int fg2_server_result(fg2_server *S, long long id, float *out, double *times, in
   if(times) { times[0] = q->t_arrive; times[1] = q->t_admit; times[2] = q->t_done; }
   if(status) *status = q->t_done < 0 ? -1 : q->status;
   if(out && q->t_done >= 0) memcpy(out, q->host.data(), q->host.size() * sizeof(float));
+  return FG2_OK;
+}
+
+int fg2_server_set_mode(fg2_server *S, int mode)
+{
+  if(!S || (mode != FG2_SERVE_BATCHED && mode != FG2_SERVE_PER_RUN)) return fail(FG2_ERR_ARG, "bad server mode");
+  if(!S->active.empty() || !S->queue.empty())
+    return fail(FG2_ERR_ARG, "the mode changes only while the server is idle");
+  S->mode = mode;
   return FG2_OK;
 }
 

@@ -1,26 +1,26 @@
-// driver.cpp -- the fg2 command-line driver over the C ABI (fg2run).
+// driver.cpp -- command-line front end over the C ABI (fg2run).
 //
 //   fg2run MODULE [--help] [-b p1 [p2]] [-c C] [-d dev] [-l T [L1 [L2]]]
 //          [-n frames [N1 [N2]]] [name=val ...] [seed=S] [INPUT]
 //
-// Behaviour of the reference's main() (src/main.cpp:29-120), solve()
-// (src/solve.cpp:25-64), .raw snapshots (src/io.cpp:24-76) and initial
-// conditions (src/init.cpp:26-42, src/*/init.h), with the module chosen at
-// run time instead of at build time (Makefile:19-22):
-//   - INPUT that names an existing file is loaded as a snapshot and the run
-//     continues from its frame number (the first digit run of the name);
-//     otherwise INPUT names an initial condition and frame 0000.raw is
-//     written first;
-//   - each frame advances the time by T / frames with
-//     dt = min(target - t, getdt()) and writes NNNN.raw;
-//   - a step that produces NaN writes ****.raw and exits with status 255.
-// Initial conditions, snapshots and steps go through the C ABI (fg2_init,
-// fg2_load, fg2_dump, fg2_step); the reference's libc rand() in KH and Kf is
-// replaced by fg2_init's seeded stream (seed=S, default 0).  The start time
-// read from a snapshot stays in FP64 (the reference narrows it to R,
-// src/main.cpp:34,106), so a resumed run hits the same frame times as one
-// that never stopped.
+// Option meanings and the run sequence follow the reference's main()
+// (src/main.cpp:29-120) and solve() (src/solve.cpp:25-64); snapshots are the
+// reference's .raw files (src/io.cpp:24-76).  The module is an argument
+// here, where the reference builds one binary per module (Makefile:19-22).
+//   - An INPUT naming an existing file is read as a snapshot; the run goes on
+//     from the frame number of its first digit run (src/misc.cpp:48-55).
+//     Any other INPUT names an initial condition, and 0000.raw is written.
+//   - Frame k ends at t_final - (frames - k) T / frames; within a frame every
+//     step takes dt = min(frame end - t, getdt()), and each frame ends with
+//     a NNNN.raw snapshot.
+//   - A step that leaves NaN in the state writes ****.raw at that step's
+//     start time and exits with status 255.
+// The start time of a resumed run stays FP64 (the reference narrows it to
+// R, src/main.cpp:34,106), so a resumed run reaches exactly the frame times
+// of a run that never stopped.  The libc rand() of the KH and Kf initial
+// conditions is fg2_init's seeded stream (seed=S, default 0).
 #include <hip/hip_runtime_api.h>
+#include <cctype>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -34,7 +34,7 @@
 
 namespace {
 
-[[noreturn]] void die(const char *fmt, ...)
+[[noreturn]] void quit(const char *fmt, ...)
 {
   fflush(stdout);
   va_list ap;
@@ -44,178 +44,215 @@ namespace {
   exit(-1);
 }
 
-void usage(const char *bad)
+[[noreturn]] void help(const char *bad)
 {
-  if(bad) {
-    fprintf(stderr, "fg2run: illegal option \"%s\"\nTry `fg2run --help` for more information.\n", bad);
-    exit(-1);
-  }
+  if(bad)
+    quit("fg2run: illegal option \"%s\" (fg2run --help lists the options)\n", bad);
   printf("Usage: fg2run MODULE [OPTION...] [INPUT_FILE]\n"
-         "MI355X time-step path of the fg2 2D finite grid code\n\n"
-         "  MODULE            hydro, hydro-isot or disk\n"
-         "      --help        display this help and exit\n"
-         "  -b                periodic boundary conditions (p/P = periodic)\n"
-         "  -c                Courant number\n"
-         "  -d                specify device id\n"
-         "  -l                total time, box size\n"
-         "  -n                number of frames and grids\n"
-         "  parameter=val     set parameters in a particular scheme\n"
-         "  seed=S            seed of the random initial conditions\n");
+         "Advance a 2D grid of the chosen physics module on an MI355X and write\n"
+         "a .raw snapshot per frame.\n\n"
+         "  MODULE             hydro | hydro-isot | disk\n"
+         "  --help             print this text\n"
+         "  -b P1 [P2]         boundary of dim 1 [and dim 2]: p or P periodic, else closed\n"
+         "  -c C               Courant factor of the time step\n"
+         "  -d DEV             HIP device index\n"
+         "  -l T [L1 [L2]]     simulated time span [and box lengths]\n"
+         "  -n F [N1 [N2]]     frames to write [and grid cells per side]\n"
+         "  NAME=VALUE         module parameter (gamma=1.4, nus=1e-4, ...)\n"
+         "  seed=S             seed of the random part of an initial condition\n"
+         "  INPUT_FILE         snapshot to resume from, or the initial condition's name\n");
   exit(0);
 }
 
-void hip_check(hipError_t e, const char *what)
+void need(hipError_t e, const char *what)
 {
-  if(e != hipSuccess) die("%s: %s\n", what, hipGetErrorString(e));
+  if(e != hipSuccess) quit("%s failed: %s\n", what, hipGetErrorString(e));
 }
 
-bool exists(const char *name)
+bool file_exists(const char *name)
 {
-  FILE *f = fopen(name, "r");
-  if(!f) return false;
-  fclose(f);
-  return true;
+  if(FILE *f = fopen(name, "r")) { fclose(f); return true; }
+  return false;
 }
 
-int frame_of(const char *name)                             // src/misc.cpp:48-55
+// frame number of a snapshot name: its first run of digits (src/misc.cpp:48-55)
+int frame_number(const char *name)
 {
-  for(const char *c = name; *c; ++c)
-    if('0' <= *c && *c <= '9') return atoi(c);
-  return 0;
+  const char *d = name;
+  while(*d && (*d < '0' || *d > '9')) ++d;
+  return *d ? atoi(d) : 0;
 }
 
-std::string frame_name(int i)
+std::string snapshot_name(int frame)
 {
-  char b[64];
-  snprintf(b, sizeof b, "%04d.raw", i);
+  char b[32];
+  snprintf(b, sizeof b, "%04d.raw", frame);
   return b;
 }
 
-bool is_option(const char *a) { return a[0] == '-' && ((a[1] >= 'A' && a[1] <= 'Z') || (a[1] >= 'a' && a[1] <= 'z')); }
+// an argument that begins a new option: '-' followed by a letter
+bool starts_option(const char *a) { return a[0] == '-' && isalpha((unsigned char)a[1]); }
+
+struct Options {
+  std::string module;
+  const char *input = "default";
+  int frames = 10, n1 = 1024, n2 = 1024, device = 0, p1 = 1, p2 = 1;   // src/main.cpp:33-34
+  double span = 1.0, L1 = 1.0, L2 = 1.0, courant = 0.5;               // src/global.cpp:28-29
+  uint64_t seed = 0;
+  std::vector<const char *> assignments;
+};
+
+Options parse(int argc, char **argv)                        // src/main.cpp:41-60
+{
+  Options o;
+  o.module = argv[1];
+  if(o.module != "hydro" && o.module != "hydro-isot" && o.module != "disk")
+    quit("fg2run: unknown module \"%s\"; the modules are hydro, hydro-isot and disk\n",
+         o.module.c_str());
+  // values that follow option argv[i]: up to `most`, stopping at the next option
+  auto values = [&](int &i, int most) {
+    std::vector<const char *> v;
+    while((int)v.size() < most && i + 1 < argc && !starts_option(argv[i + 1])) v.push_back(argv[++i]);
+    return v;
+  };
+  auto periodic = [](const char *s) { return s[0] == 'p' || s[0] == 'P'; };
+  for(int i = 2; i < argc; ++i) {
+    const char *a = argv[i];
+    if(strchr(a, '=')) {
+      if(!strncmp(a, "seed=", 5)) o.seed = strtoull(a + 5, nullptr, 10);
+      else o.assignments.push_back(a);
+      continue;
+    }
+    if(a[0] != '-') { o.input = a; continue; }
+    std::vector<const char *> v;
+    switch(a[1]) {
+    case 'b':
+      v = values(i, 2);
+      if(v.empty()) help(a);
+      o.p1 = periodic(v[0]);
+      if(v.size() > 1) o.p2 = periodic(v[1]);
+      break;
+    case 'c':
+      v = values(i, 1);
+      if(v.empty()) help(a);
+      o.courant = atof(v[0]);
+      break;
+    case 'd':
+      v = values(i, 1);
+      if(v.empty()) help(a);
+      o.device = atoi(v[0]);
+      break;
+    case 'l':
+      v = values(i, 3);
+      if(v.empty()) help(a);
+      o.span = atof(v[0]);
+      if(v.size() > 1) o.L1 = o.L2 = atof(v[1]);
+      if(v.size() > 2) o.L2 = atof(v[2]);
+      break;
+    case 'n':
+      v = values(i, 3);
+      if(v.empty()) help(a);
+      o.frames = atoi(v[0]);
+      if(v.size() > 1) o.n1 = o.n2 = atoi(v[1]);
+      if(v.size() > 2) o.n2 = atoi(v[2]);
+      break;
+    default:
+      help(a);
+    }
+  }
+  return o;
+}
+
+const char *plural(int k) { return k == 1 ? "" : "s"; }
 
 } // namespace
 
 int main(int argc, char **argv)
 {
   for(int i = 1; i < argc; ++i)
-    if(!strcmp(argv[i], "--help")) usage(nullptr);
-  if(argc < 2) usage(nullptr);
-  const std::string module = argv[1];
-  if(module != "hydro" && module != "hydro-isot" && module != "disk")
-    die("fg2run: unknown module \"%s\" (hydro, hydro-isot, disk)\n", module.c_str());
+    if(!strcmp(argv[i], "--help")) help(nullptr);
+  if(argc < 2) help(nullptr);
+  const Options o = parse(argc, argv);
 
-  const char *input = "default";
-  int n0 = 10, n1 = 1024, n2 = 1024, dev = 0, p1 = 1, p2 = 1;
-  double l0 = 1.0, L1 = 1.0, L2 = 1.0, courant = 0.5;
-  uint64_t seed = 0;
-  std::vector<const char *> paras;
-  auto more = [&](int i) { return i + 1 < argc && !is_option(argv[i + 1]); };
-  for(int i = 2; i < argc; ++i) {                           // src/main.cpp:41-60
-    const char *a = argv[i];
-    if(strchr(a, '=')) {
-      if(!strncmp(a, "seed=", 5)) seed = strtoull(a + 5, nullptr, 10);
-      else paras.push_back(a);
-    } else if(a[0] != '-') {
-      input = a;
-    } else {
-      switch(a[1]) {
-      case 'b':
-        if(!more(i)) usage(a);
-        p1 = (argv[i + 1][0] == 'p' || argv[i + 1][0] == 'P'); ++i;
-        if(more(i)) { p2 = (argv[i + 1][0] == 'p' || argv[i + 1][0] == 'P'); ++i; }
-        break;
-      case 'c': if(!more(i)) usage(a); courant = atof(argv[++i]); break;
-      case 'd': if(!more(i)) usage(a); dev = atoi(argv[++i]); break;
-      case 'l':
-        if(!more(i)) usage(a);
-        l0 = atof(argv[++i]);
-        if(more(i)) { L1 = L2 = atof(argv[++i]); if(more(i)) L2 = atof(argv[++i]); }
-        break;
-      case 'n':
-        if(!more(i)) usage(a);
-        n0 = atoi(argv[++i]);
-        if(more(i)) { n1 = n2 = atoi(argv[++i]); if(more(i)) n2 = atoi(argv[++i]); }
-        break;
-      default: usage(a);
-      }
-    }
-  }
-
-  printf("2D finite grid code, MI355X time-step path (module %s)\n\n", module.c_str());
-  int count = 0;
-  if(hipGetDeviceCount(&count) != hipSuccess || dev >= count) die("  Device %d/%d   : does not exist, QUIT\n", dev, count);
-  if(hipSetDevice(dev) != hipSuccess) die("fail to pick device, QUIT\n");
+  printf("fg2run: module %s on an MI355X\n", o.module.c_str());
+  int ndev = 0;
+  if(hipGetDeviceCount(&ndev) != hipSuccess || o.device < 0 || o.device >= ndev)
+    quit("fg2run: no HIP device %d (%d visible)\n", o.device, ndev);
+  need(hipSetDevice(o.device), "hipSetDevice");
   hipDeviceProp_t prop;
-  hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-  printf("  Device %d/%d   : \"%s\" with %gMiB global and %gKiB shared memory\n", dev, count,
-         prop.name, prop.totalGlobalMem / 1048576.0, prop.sharedMemPerBlock / 1024.0);
+  need(hipGetDeviceProperties(&prop, o.device), "hipGetDeviceProperties");
+  printf("  device       %d of %d, %s, %.0f MiB\n", o.device, ndev, prop.name,
+         prop.totalGlobalMem / 1048576.0);
 
   fg2_run *run = nullptr;
   size_t bytes = 0;
-  printf("  Resolution   : %d x %d", n1, n2);
-  if(fg2_setup(&run, module.c_str(), n1, n2, &bytes)) die(", fail to allocate memory, QUIT (%s)\n", fg2_last_error());
-  printf(" using %.3gMiB (%.3g%%) of global memory\n", bytes / 1048576.0, 100.0 * bytes / prop.totalGlobalMem);
-  if(fg2_set_geometry(run, L1, L2, p1, p2, courant)) die("%s, QUIT\n", fg2_last_error());
-  for(const char *p : paras) {
-    if(const char *r = fg2_para(run, p)) printf("  Parameter    : %s\n", r);
-    else printf("  Fail to set  : \"%s\"\n", p);
+  if(fg2_setup(&run, o.module.c_str(), o.n1, o.n2, &bytes))
+    quit("fg2run: cannot set up a %d x %d grid: %s\n", o.n1, o.n2, fg2_last_error());
+  printf("  grid         %d x %d cells, %.1f MiB of device memory\n", o.n1, o.n2,
+         bytes / 1048576.0);
+  if(fg2_set_geometry(run, o.L1, o.L2, o.p1, o.p2, o.courant))
+    quit("fg2run: %s\n", fg2_last_error());
+  for(const char *a : o.assignments) {
+    if(const char *r = fg2_para(run, a)) printf("  parameter    %s\n", r);
+    else printf("  ignored      \"%s\" (%s)\n", a, fg2_last_error());
   }
 
-  int i;
+  int frame;
   double t = 0.0;
-  if(exists(input)) {                                        // src/main.cpp:104-115
-    printf("  Input file   : \"%s\"...", input);
-    fflush(stdout);
-    i = frame_of(input);
-    if(fg2_load(run, input, &t)) die("\n%s\n", fg2_last_error());
+  if(file_exists(o.input)) {                                 // src/main.cpp:104-115
+    frame = frame_number(o.input);
+    if(fg2_load(run, o.input, &t)) quit("fg2run: %s\n", fg2_last_error());
     fg2_config(run);
-    printf("\b\b\b, start at frame %d, time %g\n", i, t);
+    printf("  resume       \"%s\": frame %d, t = %g\n", o.input, frame, t);
   } else {
-    printf("  Initialize   : \"%s\"\n", input);
     fg2_config(run);
-    if(fg2_init(run, input, seed)) die("init: %s\n", fg2_last_error());
-    i = 0;
-    if(fg2_dump(run, frame_name(i).c_str(), t)) die("dump: %s\n", fg2_last_error());
+    if(fg2_init(run, o.input, o.seed)) quit("fg2run: initial condition: %s\n", fg2_last_error());
+    frame = 0;
+    printf("  initial      \"%s\"\n", o.input);
+    if(fg2_dump(run, snapshot_name(frame).c_str(), t)) quit("fg2run: %s\n", fg2_last_error());
   }
+  printf("  span         %g in %d frame%s\n", o.span, o.frames, plural(o.frames));
 
-  printf("  Time         : %g with %d frame%s\n", l0, n0, n0 > 1 ? "s" : "");
-  double flops, bits, alg;
+  double flops = 0, bits = 0, alg = 0;
   fg2_perf_model(run, &flops, &bits, &alg);
-  const double t_final = t + l0;
-  const int n = i + n0;
-  printf("================================ Start Simulation ==============================\n");
+  hipStream_t st = (hipStream_t)fg2_stream(run);
   hipEvent_t e0, e1;
-  hip_check(hipEventCreate(&e0), "hipEventCreate");
-  hip_check(hipEventCreate(&e1), "hipEventCreate");
-  for(const double dt_dump = (t_final - t) / (n - i); i++ < n;) {   // src/solve.cpp:32-58
-    const double target = t_final - (n - i) * dt_dump;
-    int m = 0;
-    printf("%4d:%7.2f ->%7.2f", i, t, target);
+  need(hipEventCreate(&e0), "hipEventCreate");
+  need(hipEventCreate(&e1), "hipEventCreate");
+  const double t_final = t + o.span;
+  const int last = frame + o.frames;
+  const double frame_span = (t_final - t) / o.frames;
+  printf("-- run ------------------------------------------------------------------\n");
+  while(frame < last) {                                      // src/solve.cpp:32-58
+    ++frame;
+    const double end = t_final - (last - frame) * frame_span;
+    printf("  frame %4d  t %9.4f -> %9.4f", frame, t, end);
     fflush(stdout);
-    hip_check(hipEventRecord(e0, (hipStream_t)fg2_stream(run)), "hipEventRecord");
-    while(t < target) {
-      const double dt = std::fmin(target - t, fg2_getdt(run));
-      ++m;
+    int steps = 0;
+    need(hipEventRecord(e0, st), "hipEventRecord");
+    for(; t < end; ++steps) {
+      const double dt = std::fmin(end - t, fg2_getdt(run));
       if(int err = fg2_step(run, t, dt)) {
-        printf(" crashed in %d step%s\n", m, m > 1 ? "s" : "");
-        if(fg2_dump(run, "****.raw", t)) fprintf(stderr, "dump: %s\n", fg2_last_error());
-        printf("================================ Abort Simulation ==============================\n");
-        die("ERROR: %s\n", err == FG2_ERR_NAN ? "the state contains NaN" : fg2_last_error());
+        printf("  FAILED at step %d of the frame\n", steps + 1);
+        if(fg2_dump(run, "****.raw", t)) fprintf(stderr, "fg2run: %s\n", fg2_last_error());
+        printf("-- stopped: state written to ****.raw ------------------------------------\n");
+        quit("fg2run: %s\n", err == FG2_ERR_NAN ? "the state contains NaN" : fg2_last_error());
       }
-      t = std::fmin(t + dt, target);
+      t = std::fmin(t + dt, end);
     }
-    hip_check(hipEventRecord(e1, (hipStream_t)fg2_stream(run)), "hipEventRecord");
-    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    need(hipEventRecord(e1, st), "hipEventRecord");
+    need(hipEventSynchronize(e1), "hipEventSynchronize");
     float ms = 0;
-    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-    const double ns = ms * 1.0e6 / (m > 0 ? m : 1);
-    printf(", %d step%s (%.3fms/step, %.3fGflops, %.3fGbps)\n", m, m > 1 ? "s" : "",
-           1.0e-6 * ns, flops / ns, bits / ns);
-    if(fg2_dump(run, frame_name(i).c_str(), t)) die("dump: %s\n", fg2_last_error());
+    need(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    const double s_per_step = 1e-3 * ms / (steps > 0 ? steps : 1);
+    // the reference's model rates (src/config.cpp:30-36) beside the step time
+    printf("  %d step%s, %.3f ms/step, model %.1f GFLOP/s %.1f GB/s\n", steps, plural(steps),
+           1e3 * s_per_step, 1e-9 * flops / s_per_step, 1e-9 * bits / 8 / s_per_step);
+    if(fg2_dump(run, snapshot_name(frame).c_str(), t)) quit("fg2run: %s\n", fg2_last_error());
   }
-  printf("================================ Done  Simulation ==============================\n");
+  printf("-- done -----------------------------------------------------------------\n");
   fg2_free(run);
-  hip_check(hipEventDestroy(e1), "hipEventDestroy");
-  hip_check(hipEventDestroy(e0), "hipEventDestroy");
+  need(hipEventDestroy(e1), "hipEventDestroy");
+  need(hipEventDestroy(e0), "hipEventDestroy");
   return 0;
 }

@@ -108,3 +108,42 @@ def disk_kepler(n1, n2, seed=4, l1=None, rin=3.0, gm=1.0, h_over_r=0.05,
     x[..., 4] = np.log(e)
     x[..., 5] = np.log(rad_frac * e)
     return x
+
+
+def log_uniform_sizes(rng, count, lo, hi, mult):
+    """Grid sides drawn log-uniform on [lo, hi], rounded to a multiple of mult."""
+    v = np.exp(rng.uniform(np.log(lo), np.log(hi), count))
+    return np.clip((np.round(v / mult) * mult).astype(int), lo, hi)
+
+
+def batched_workload(nruns=64, seed=7):
+    """The batched workload: nruns hydro-isot runs with sides n1, n2 drawn
+    independently log-uniform on [128, 1024] as multiples of 32, each
+    starting from a seeded density perturbation (isot_density, rms 0.05,
+    |k| <= 8, u = 0).  Returns (n1s, n2s, states)."""
+    rng = np.random.default_rng(seed)
+    n1s = log_uniform_sizes(rng, nruns, 128, 1024, 32)
+    n2s = log_uniform_sizes(rng, nruns, 128, 1024, 32)
+    xs = [isot_density(int(a), int(b), seed=seed + 1 + k) for k, (a, b) in enumerate(zip(n1s, n2s))]
+    return n1s, n2s, xs
+
+
+def serving_workload(nreq=1000, rate=100.0, seed=11):
+    """The served request stream: Poisson arrivals at `rate` per second,
+    module hydro or hydro-isot with equal odds, sides log-uniform on
+    [64, 2048], step counts uniform on [50, 2000].  Returns (arrivals,
+    modules, n1s, n2s, steps, state) where state(k) builds request k's
+    seeded initial state (hydro_random or isot_density, FP32)."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    mods = ["hydro" if m == 0 else "hydro-isot" for m in rng.integers(0, 2, nreq)]
+    n1s = log_uniform_sizes(rng, nreq, 64, 2048, 1)
+    n2s = log_uniform_sizes(rng, nreq, 64, 2048, 1)
+    steps = rng.integers(50, 2001, nreq)
+
+    def state(k):
+        a, b = int(n1s[k]), int(n2s[k])
+        if mods[k] == "hydro":
+            return hydro_random(a, b, seed=1000 + k, dtype=np.float32)
+        return isot_density(a, b, seed=1000 + k, dtype=np.float32)
+    return arrivals, mods, n1s, n2s, steps, state

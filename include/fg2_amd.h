@@ -305,6 +305,14 @@ int fg2_server_result(fg2_server *server, long long id, float *out, double *time
 /* Total cell-updates launched so far and the device bytes held by runs. */
 int fg2_server_stats(fg2_server *server, long long *cell_steps, size_t *bytes_in_use);
 
+/* How a tick advances the active runs: FG2_SERVE_BATCHED (default), one
+ * batched one-pass launch per module; FG2_SERVE_PER_RUN, each run's own
+ * one-pass step launch (the shape fg2_step gives a run of its size) on one
+ * of 8 streams -- the baseline the batched mode is measured against, with
+ * the same admission, arena and copies.  Changes only while idle. */
+enum { FG2_SERVE_BATCHED = 0, FG2_SERVE_PER_RUN = 1 };
+int fg2_server_set_mode(fg2_server *server, int mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -229,6 +229,17 @@ template<class R> void rhs_isot(const Sim<R> &S, const Stencil<R> &st,
   for(int q = 0; q < 3; ++q) out[q] = dt[q];
 }
 
+// Flux limiter of the radiative diffusion, src/disk/eqns.cuh:194-196,216:
+// Lambda(R) = (2 + R) / (6 + 3 R + R^2) and its derivative
+// Lambda'(R) = -R (4 + R) / (6 + 3 R + R^2)^2, in the reference's order of
+// operations.
+template<class R> void flux_limiter(R bigR, R &Lambda, R &Lambdap)
+{
+  const R denom = (R)6 + ((R)3 + bigR) * bigR;
+  Lambda = ((R)2 + bigR) / denom;
+  Lambdap = -bigR * ((R)4 + bigR) / (denom * denom);
+}
+
 // disk: variables (lnd, ur, uz, Omg, lne, lner) on the (log r, theta) plane;
 // src/disk/eqns.cuh:38-247
 template<class R> void rhs_disk(const Sim<R> &S, const Stencil<R> &st,
@@ -332,8 +343,8 @@ template<class R> void rhs_disk(const Sim<R> &S, const Stencil<R> &st,
     const R chi = es * den + kappa;
     const R d_lner_2 = dr[LNER] * dr[LNER] + dz[LNER] * dz[LNER];
     const R bigR = std::sqrt(d_lner_2) / chi;
-    const R denom = (R)6 + ((R)3 + bigR) * bigR;
-    const R Lambda = ((R)2 + bigR) / denom;
+    R Lambda, Lambdap;
+    flux_limiter(bigR, Lambda, Lambdap);
     dt[UR] -= Lambda * rad * (dr[LND] + dr[LNE]);
     dt[UZ] -= Lambda * rad * (dz[LND] + dz[LNE]);
 
@@ -345,7 +356,6 @@ template<class R> void rhs_disk(const Sim<R> &S, const Stencil<R> &st,
     const R d_lnd_d_lner = dr[LND] * dr[LNER] + dz[LND] * dz[LNER];
     const R lap_er_er = lap[LNER] + d_lner_2;
     const R nu_rad = Lambda / chi;
-    const R Lambdap = -bigR * ((R)4 + bigR) / (denom * denom);
     const R dr_lnEr = dr[LND] + dr[LNER];
     const R dz_lnEr = dz[LND] + dz[LNER];
     const R drr_lnEr = drr_lnd + drr_lner;
@@ -917,6 +927,14 @@ int orc_run(void *p, double t, double dt, int n)
     for(int k = 0; k < n && !r; ++k) r = step(S, t + k * dt, dt);
     return r;
   });
+}
+
+// The disk flux limiter Lambda(R) and Lambda'(R) in FP64 (fp64 != 0) or
+// FP32, as rhs_disk evaluates them.
+void orc_flux_limiter(double bigR, int fp64, double *out)
+{
+  if(fp64) { double a, b; flux_limiter<double>(bigR, a, b); out[0] = a; out[1] = b; }
+  else { float a, b; flux_limiter<float>((float)bigR, a, b); out[0] = a; out[1] = b; }
 }
 
 double orc_getdt(void *p) { return dispatch((OrcHandle *)p, [](auto &S) { return getdt(S); }); }

@@ -28,6 +28,10 @@ Pinning (which test fixes each function against something other than itself):
       uniform state -> gravity only, Keplerian balance of the radial
       equation, manufactured smooth fields against sympy's continuous RHS
       (chain rule from (log r, theta) to (R, z))           tests/test_oracle_eqns.py
+  disk flux limiter Lambda(R), Lambda'(R) (src/disk/eqns.cuh:194-196,216)
+      Lambda(0) = 1/3 (Eddington), R Lambda(R) -> 1 (free streaming),
+      Lambda' against a central difference and its closed-form limits
+      0 and -1/R^2                                     tests/test_oracle_eqns.py
   first_order / Euler (src/*/Euler.cuh)
       hydro: identity.  disk: the quartic root x^4 + x - x0 = 0 by
       substitution.  hydro-isot forcing, for the phases drawn: kick along
@@ -109,6 +113,7 @@ def lib():
             "orc_nan_flag": (I, [P]),
             "orc_init": (None, [P, ctypes.c_char_p, ctypes.c_ulonglong]),
             "orc_splitmix64": (ctypes.c_ulonglong, [ctypes.POINTER(ctypes.c_ulonglong)]),
+            "orc_flux_limiter": (None, [D, I, DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -248,3 +253,11 @@ def splitmix64(state, count):
     oracle's stand-in for libc rand())."""
     st = ctypes.c_ulonglong(state)
     return [lib().orc_splitmix64(ctypes.byref(st)) for _ in range(count)]
+
+
+def flux_limiter(bigR, fp64=True):
+    """(Lambda, Lambda') of the disk's radiative flux limiter at R, as the
+    oracle's disk right-hand side evaluates them."""
+    out = (ctypes.c_double * 2)()
+    lib().orc_flux_limiter(float(bigR), 1 if fp64 else 0, out)
+    return out[0], out[1]

@@ -118,11 +118,14 @@ def test_config2_kh_3584_100_steps(fg2):
     """The benchmark workload itself: hydro KH 3584^2 periodic, FP32, the
     CFL dt of the initial state, 100 steps of the one-pass step kernel
     against 100 steps of the FP64 oracle from the same FP32 state.  Bars:
-    lnd and lne within 1e-5 relative L2; the velocity vector (u1, u2) within
-    1e-5 relative L2 of its own norm.  u2 alone is 50x smaller than u1 (the
-    0.01 noise against the +-0.5 shear) and carries errors at the level of
-    u1's FP32 rounding, so its own-norm relative L2 is bounded at 2e-5
-    (measured 1.15e-5)."""
+    lnd, u1 and lne within 1e-5 relative L2; the velocity vector (u1, u2)
+    within 1e-5 relative L2 of its own norm.  u2 alone is 50x smaller than
+    u1 (the 0.01 noise against the +-0.5 shear): the oracle's own FP32
+    instantiation, stepped from the same state with the same dt, is 1.31e-5
+    from its FP64 one on u2 (tests/golden/fp32_floor_100_steps.json, made
+    by tests/fp32_floor.py), so 1e-5 is below what FP32 arithmetic allows
+    on that field.  Its bar is 1.25 x that floor, and every field must also
+    stay within 1.25 x its own floor."""
     from oracle.oracle import Oracle
     from fg2_amd import ics
     n, steps = 3584, 100
@@ -142,7 +145,21 @@ def test_config2_kh_3584_100_steps(fg2):
     e2 = rel_l2(y, z)
     ev = np.sqrt(np.sum((y[..., 1:3] - z[..., 1:3]) ** 2) / np.sum(z[..., 1:3] ** 2))
     _say("config2 rel L2 per field", e2, "Linf", rel_linf(y, z), "velocity", ev)
-    assert e2[0] < 1e-5 and e2[3] < 1e-5 and ev < 1e-5 and e2[1] < 1e-5 and e2[2] < 2e-5, (e2, ev)
+    floor = np.array(list(_fp32_floor("kh").values()))
+    _say("FP32 floor of the oracle", floor)
+    assert e2[0] < 1e-5 and e2[3] < 1e-5 and ev < 1e-5 and e2[1] < 1e-5, (e2, ev)
+    assert np.all(e2 < 1.25 * floor), (e2, floor)
+
+
+def _fp32_floor(which):
+    """Per-field relative L2 of the oracle's FP32 instantiation against its
+    FP64 one after 100 steps of a workload (tests/fp32_floor.py)."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "fp32_floor_100_steps.json")
+    with open(path) as f:
+        return json.load(f)[which]["rel_l2_fp32_vs_fp64"]
 
 
 def test_nan_detection(fg2):

@@ -118,3 +118,29 @@ def test_kolmogorov_forcing():
     i = np.arange(n)[:, None] * np.ones((1, n))
     np.testing.assert_allclose(F[..., 2], 0.2 * np.cos(-3.0 * i * d1), rtol=1e-12)
     assert np.abs(F[..., :2]).max() == 0.0
+
+
+def test_disk_flux_limiter_limits():
+    """The radiative flux limiter of the disk module, Lambda(R) =
+    (2 + R) / (6 + 3R + R^2) (src/disk/eqns.cuh:194-196), is pinned by its
+    physical limits, not by a second restatement: the diffusion (Eddington)
+    limit Lambda(0) = 1/3, the free-streaming limit R Lambda(R) -> 1, and
+    Lambda <= 1/3 throughout.  Its derivative (src/disk/eqns.cuh:216) is
+    checked against a central difference of Lambda and against its own
+    limits, Lambda'(0) = 0 and R^2 Lambda'(R) -> -1."""
+    from oracle.oracle import flux_limiter
+    lam0, dlam0 = flux_limiter(0.0)
+    assert lam0 == 1.0 / 3.0 and dlam0 == 0.0
+    for R in (1e6, 1e8, 1e10):
+        lam, dlam = flux_limiter(R)
+        assert abs(R * lam - 1.0) < 2.0 / R          # R Lambda = 1 - 1/R + O(1/R^2)
+        assert abs(R * R * dlam + 1.0) < 3.0 / R     # R^2 Lambda' = -1 + 2/R + O(1/R^2)
+    for R in (1e-3, 0.1, 0.7, 1.0, 3.0, 25.0):
+        lam, dlam = flux_limiter(R)
+        assert 0.0 < lam <= 1.0 / 3.0 and dlam < 0.0
+        h = 1e-5 * max(R, 1.0)
+        fd = (flux_limiter(R + h)[0] - flux_limiter(R - h)[0]) / (2 * h)
+        assert abs(fd - dlam) < 1e-8 * max(1.0, abs(dlam)), (R, fd, dlam)
+    # the FP32 instantiation rounds the same expression
+    lam32, _ = flux_limiter(0.5, fp64=False)
+    assert abs(lam32 - flux_limiter(0.5)[0]) < 1e-7

@@ -113,6 +113,7 @@ def _load():
         "fg2_batch_graph_run": (I, [P, DP, I, I]),
         "fg2_batch_stream": (P, [P]),
         "fg2_batch_autotune": (I, [P, DP, IP, IP]),
+        "fg2_batch_step_static": (I, [PP, I, DP, I, I]),
         "fg2_server_create": (I, [PP, ctypes.c_size_t, I, I]),
         "fg2_server_free": (None, [P]),
         "fg2_server_submit": (I, [P, ctypes.c_char_p, I, I, I, FP, D,
@@ -449,6 +450,20 @@ class Batch:
         a, b = ctypes.c_int(), ctypes.c_int()
         _check(lib.fg2_batch_autotune(self.h, _dp(d), ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+def batch_step_static(runs, dts, waves=1, tile_h=0):
+    """One batched step of `runs` through fg2_batch_step_static (no device
+    allocation; at most FG2_STATIC_MAX_RUNS runs); returns OK or ERR_NAN."""
+    arr = (ctypes.c_void_p * len(runs))(*[r.h.value for r in runs])
+    d = np.ascontiguousarray(dts, dtype=np.float64)
+    code = lib.fg2_batch_step_static(arr, len(runs), _dp(d), int(waves), int(tile_h))
+    if code not in (OK, ERR_NAN):
+        _check(code)
+    return code
+
+
+STATIC_MAX_RUNS = 512
 
 
 class Server:

@@ -1400,6 +1400,140 @@ int fg2_batch_graph_run(fg2_batch *b, const double *dts, int steps, int replays)
 } // extern "C"
 
 // ---------------------------------------------------------------------------
+// The batched step without any allocation (fg2_batch_step_static).
+//
+// fg2_batch_step's kernels read per-run launch records and tile-queue
+// counters that fg2_batch_create allocates.  Here they live in `scratch`, a
+// file-scope device array of fixed size that every GPU the library runs on
+// holds until the process ends: 3 x FG2_STATIC_MAX_RUNS records (one set
+// per substage) and the counters.  Its device address is taken once per
+// device with hipGetSymbolAddress on the first call there (the array's name
+// in host code is not a device address), and every later use goes through
+// that pointer.  All calls on one GPU share the array, so calls that
+// overlap on one GPU race; a mutex serialises calls from one process.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int STATIC_MAX_RUNS = FG2_STATIC_MAX_RUNS;
+constexpr size_t STATIC_REC_BYTES = 3 * (size_t)STATIC_MAX_RUNS * sizeof(BatchRun);
+constexpr size_t STATIC_BYTES = STATIC_REC_BYTES + 256;      // + the tile counters
+
+static __device__ __align__(256) unsigned char scratch[STATIC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_static_mu;
+unsigned char *g_scratch_ptr[MAX_DEVICES] = {};              // scratch's address per device
+BatchRun g_static_host[3][STATIC_MAX_RUNS];                  // host side of the records
+
+// scratch's device address on the current device (looked up on the first
+// call there); the caller holds g_static_mu
+int static_scratch(unsigned char **out)
+{
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if(dev < 0 || dev >= MAX_DEVICES) return fail(FG2_ERR_ARG, "device %d out of range", dev);
+  if(!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (unsigned char *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FG2_OK;
+}
+
+} // namespace
+
+extern "C" int fg2_batch_step_static(fg2_run *const *runs, int nruns, const double *dts,
+                                     int waves, int tile_h)
+{
+  if(!runs || !dts || nruns < 1) return fail(FG2_ERR_ARG, "bad batch arguments");
+  if(nruns > STATIC_MAX_RUNS)
+    return fail(FG2_ERR_SHAPE, "%d runs: fg2_batch_step_static takes at most %d", nruns,
+                STATIC_MAX_RUNS);
+  if(waves != 1 && waves != 2 && waves != 4) return fail(FG2_ERR_SHAPE, "waves must be 1, 2 or 4");
+  const int module = runs[0] ? runs[0]->module : 0;
+  bool step = true;
+  for(int i = 0; i < nruns; ++i) {
+    fg2_run *r = runs[i];
+    if(int e = check_run(r)) return e;
+    if(r->module != module) return fail(FG2_ERR_ARG, "a batch holds runs of one module");
+    if(r->slab_n1g) return fail(FG2_ERR_ARG, "a batch holds whole grids, not slabs");
+    step = step && r->p1 && r->p2 && r->module != DISK;
+  }
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  unsigned char *base = nullptr;
+  if(int e = static_scratch(&base)) return e;
+  int dev = 0, cus = 256;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // order after every run's queued work (uploads), as fg2_batch_step does
+  for(int i = 0; i < nruns; ++i) {
+    fg2_run *r = runs[i];
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    randomize(r);
+    if(first_order_active(r))
+      return fail(FG2_ERR_ARG, "batched runs with an active first_order are not supported");
+  }
+  // launch records, cut as fg2_batch_create + batch_fill cut them
+  const int nstage = step ? 1 : 3;
+  int total = 0;
+  for(int k = 0; k < nstage; ++k) {
+    int begin = 0;
+    for(int i = 0; i < nruns; ++i) {
+      fg2_run *r = runs[i];
+      BatchRun &R = g_static_host[k][i];
+      R = BatchRun{};
+      R.s = r->s;
+      R.P = make_params(r);
+      R.diag = r->diag;
+      R.v = r->in[2];
+      if(step) {
+        R.tile_h = std::min(tile_h > 0 ? tile_h : 64, r->n1);
+        R.uin = r->in[r->cur];
+        R.uout = r->in[r->cur ^ 1];
+        R.ntiles = (int)step_launch_tiles(r->n1, r->n2, R.tile_h, waves, &R.nstrips);
+        for(int q = 0; q < 3; ++q) R.g[q] = (float)(dts[i] * RK_GAMMA[q]);
+      } else {
+        R.tile_h = std::min(tile_h > 0 ? tile_h : shape_of(r).tile_h, r->n1);
+        R.uin = r->in[r->cur ^ (k & 1)];
+        R.uout = r->in[r->cur ^ 1 ^ (k & 1)];
+        R.nstrips = (r->n2 + STRIP - 1) / STRIP;
+        R.ntiles = R.nstrips * ((r->n1 + R.tile_h - 1) / R.tile_h);
+        R.beta = (float)RK_BETA[k];
+        R.gdt = (float)(dts[i] * RK_GAMMA[k]);
+      }
+      R.tile_begin = begin;
+      begin += R.ntiles;
+    }
+    total = begin;
+  }
+  hipStream_t st = runs[0]->stream;
+  BatchRun *rec = (BatchRun *)base;
+  int *counters = (int *)(base + STATIC_REC_BYTES);
+  for(int k = 0; k < nstage; ++k)
+    HIP_TRY(hipMemcpyAsync(rec + (size_t)k * STATIC_MAX_RUNS, g_static_host[k],
+                           nruns * sizeof(BatchRun), hipMemcpyHostToDevice, st));
+  zero_ints_kernel<<<1, 64, 0, st>>>(counters, 3);      // the queues start at tile 0
+  const int grid = batch_grid(module, step, waves, cus);
+  if(step) {
+    HIP_TRY(launch_batch_step(module, rec, nruns, total, waves, grid, counters, st));
+  } else {
+    for(int k = 0; k < 3; ++k)
+      HIP_TRY(launch_batch_substage(module, k, rec + (size_t)k * STATIC_MAX_RUNS, nruns, total,
+                                    waves, grid, counters + k, 0, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  int status = FG2_OK;
+  for(int i = 0; i < nruns; ++i) {
+    runs[i]->cur ^= 1;
+    runs[i]->lam_valid = false;
+    const int e = fg2_check(runs[i]);
+    if(e && !status) status = e;
+  }
+  return status;
+}
+
+// ---------------------------------------------------------------------------
 // Serving: continuous batching of many runs of different sizes.
 //
 // Requests (module, grid, step count, initial state) queue on submit.  Each

@@ -276,6 +276,24 @@ void *fg2_batch_stream(fg2_batch *batch);
  * the same number of equal tiles; *tile_h receives h. */
 int fg2_batch_autotune(fg2_batch *batch, const double *dts, int *tile_h, int *waves);
 
+/* One synchronous batched step of runs[0..nruns), as fg2_batch_step of a
+ * batch made by fg2_batch_create(runs, nruns, waves, tile_h) -- same
+ * kernels, same tiles, same bits -- that allocates nothing: no hipMalloc,
+ * hipFree or other allocating call, on any call.  The launch records and
+ * tile counters live in a fixed device array of the library (about 384 KiB
+ * per GPU, held until the process ends), so nruns is at most
+ * FG2_STATIC_MAX_RUNS (more is FG2_ERR_SHAPE).  The runs' states stay in
+ * their own buffers.  Runs in the current device; the launch goes on
+ * runs[0]'s stream after every run's queued work, and the call returns
+ * after the step with FG2_OK or FG2_ERR_NAN as fg2_batch_step.  Calls that
+ * overlap on one GPU share the array and race (one process serialises its
+ * own calls).  No counterpart in the reference, which has no batching: for
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). */
+#define FG2_STATIC_MAX_RUNS 512
+int fg2_batch_step_static(fg2_run *const *runs, int nruns, const double *dts, int waves,
+                          int tile_h);
+
 /* ---- serving: continuous batching of runs of different sizes ---- */
 
 typedef struct fg2_server fg2_server;

@@ -21,10 +21,12 @@ GPU's share, with its halo exchange), the substage path on the headline
 workload, and the host CPU baseline (the oracle's FP32 instantiation on all
 host cores, on a bounded sample of the same workload).
 
-The other GPU legs run before the headline: in a fresh process the step
-takes about 275 us for its first ~50 launches and reaches its steady 236 us
-only after ~100 launches of sustained load (the GPU's clocks ramp up), so a
-headline of a few steps timed first would measure the ramp.
+The other GPU legs run before the headline, and the headline's own
+substage-path leg (100 steps of the same workload) is queued back to back
+with its warm-up and timed steps: after 5 ms or more without work the step
+takes 250-290 us per launch, and it returns to its steady 235 us only after
+some 100 launches (25 ms) of sustained load (the GPU's clocks ramp up), so a
+headline of 20 steps after an idle gap would time the ramp.
 """
 import argparse
 import ctypes
@@ -569,18 +571,35 @@ def bench_sweep(ev, sides=(128, 256, 512, 1024, 2048, 3584, 4096, 8192, 14336), 
             "rows": out}
 
 
-def headline(ev, run, dt, steps, warmup):
-    """The timed region of the headline: W untimed steps, then exactly K
-    steps between two events on the run's stream (wall clock around them
-    too); then 25 more launches with an event after each, for the median
-    launch time."""
+def headline(ev, run, dt, steps, warmup, tile=0, waves=4, preload=100):
+    """The timed region of the headline, on the run's stream.
+
+    First `preload` steps of the same workload on the substage path (three
+    launches per step; timed by events, reported as the substage figure),
+    queued back to back with the W untimed one-pass warm-up steps and the
+    timed region, so the GPU stays loaded up to the first timed launch: its
+    clocks drop after about 5 ms without work and take some 25 ms of
+    sustained load to come back, which the first launches of a fresh or idle
+    GPU would otherwise measure.  Then exactly K one-pass steps between two
+    events (wall clock around them too), and 25 more launches with an event
+    after each, for the median launch time."""
+    import fg2_amd
     stream = run.stream()
+    run.set_path(fg2_amd.PATH_SUBSTAGE)
+    run.set_tile(32, 4)
+    s0, s1 = ev.event(), ev.event()
+    ev.record(s0, stream)
+    for k in range(preload):
+        run.step_async(k * dt, dt)
+    ev.record(s1, stream)
+    run.set_path(fg2_amd.PATH_FUSED)
+    run.set_tile(tile, waves)
     for k in range(warmup):
         run.step_async(k * dt, dt)
-    assert run.check() == 0
     sums0 = run.sums()                                  # FP64 mass and momentum
+    assert run.check() == 0
+    substage_ms = ev.elapsed_ms(s0, s1) / preload
     e0, e1 = ev.event(), ev.event()
-    ev.sync(stream)
     t0 = time.perf_counter()
     ev.record(e0, stream)
     for k in range(steps):
@@ -598,7 +617,7 @@ def headline(ev, run, dt, steps, warmup):
         ev.record(evs[k + 1], stream)
     series = [1e3 * ev.elapsed_ms(evs[k], evs[k + 1]) for k in range(25)]
     status = max(status, run.check())
-    return wall, gpu_ms, status, sums0, sums1, series
+    return wall, gpu_ms, status, sums0, sums1, series, substage_ms
 
 
 def main():
@@ -677,7 +696,8 @@ def main():
     if not args.no_slab:
         legs["slabs"] = bench_slab(ev, dt)
 
-    wall, gpu_ms, status, sums0, sums1, series = headline(ev, run, dt, args.steps, args.warmup)
+    wall, gpu_ms, status, sums0, sums1, series, substage_ms = headline(
+        ev, run, dt, args.steps, args.warmup, tile=args.tile, waves=args.waves)
     if status != 0:
         print("NaN in the benchmark state", file=sys.stderr)
         sys.exit(1)
@@ -741,19 +761,6 @@ def main():
                     "(157.3 TFLOP/s with it)"},
     }
 
-    # the same workload on the substage path (three launches per step), for reference
-    stream = run.stream()
-    run.set_path(fg2_amd.PATH_SUBSTAGE)
-    run.set_tile(32, 4)
-    for k in range(5):
-        run.step_async(k * dt, dt)
-    ev.sync(stream)
-    t0 = time.perf_counter()
-    for k in range(100):
-        run.step_async(k * dt, dt)
-    ev.sync(stream)
-    substage_ms = 1e3 * (time.perf_counter() - t0) / 100
-    assert run.check() == 0
     out = {
         "metric": METRIC,
         "value": value,

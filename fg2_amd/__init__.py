@@ -114,10 +114,12 @@ def _load():
         "fg2_batch_stream": (P, [P]),
         "fg2_batch_autotune": (I, [P, DP, IP, IP]),
         "fg2_batch_step_static": (I, [PP, I, DP, I, I]),
+        "fg2_batch_solve": (I, [P, DP, DP, I, IP]),
         "fg2_server_create": (I, [PP, ctypes.c_size_t, I, I]),
         "fg2_server_free": (None, [P]),
         "fg2_server_submit": (I, [P, ctypes.c_char_p, I, I, I, FP, D,
                                   ctypes.POINTER(ctypes.c_longlong)]),
+        "fg2_server_submit_run": (I, [P, P, I, FP, D, ctypes.POINTER(ctypes.c_longlong)]),
         "fg2_server_tick": (I, [P, D, IP, IP, IP]),
         "fg2_server_result": (I, [P, ctypes.c_longlong, FP, DP, IP]),
         "fg2_server_stats": (I, [P, ctypes.POINTER(ctypes.c_longlong), SP]),
@@ -442,6 +444,19 @@ class Batch:
         d = np.ascontiguousarray(dts, dtype=np.float64)
         _check(lib.fg2_batch_graph_run(self.h, _dp(d), steps, replays))
 
+    def solve(self, t, t_final, max_steps=1 << 30):
+        """Advance run k from t[k] to t_final[k] with device-chosen steps
+        (each run's dt mode), as fg2_solve advances one run; returns (status,
+        times reached, steps taken)."""
+        tt = np.ascontiguousarray(t, dtype=np.float64).copy()
+        tf = np.ascontiguousarray(t_final, dtype=np.float64)
+        st = np.zeros(len(self.runs), dtype=np.int32)
+        code = lib.fg2_batch_solve(self.h, _dp(tt), _dp(tf), int(max_steps),
+                                   st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        if code not in (OK, ERR_NAN):
+            _check(code)
+        return code, tt, st
+
     def stream(self):
         return lib.fg2_batch_stream(self.h)
 
@@ -489,6 +504,16 @@ class Server:
         i = ctypes.c_longlong()
         _check(lib.fg2_server_submit(self.h, module.encode(), n1, n2, int(steps), _fp(x),
                                      float(now), ctypes.byref(i)))
+        self.shapes[i.value] = x.shape
+        return i.value
+
+    def submit_run(self, like, init, steps, now):
+        """A request shaped like the configured Run `like` (its module, grid,
+        geometry, parameters, dt mode and forcing seed)."""
+        x = np.ascontiguousarray(init, dtype=np.float32)
+        i = ctypes.c_longlong()
+        _check(lib.fg2_server_submit_run(self.h, like.h, int(steps), _fp(x), float(now),
+                                         ctypes.byref(i)))
         self.shapes[i.value] = x.shape
         return i.value
 

@@ -113,6 +113,31 @@ __global__ void restore_diag_kernel(DeviceDiag *d, unsigned int flags)
   d->lam_diff_bits = 0;
 }
 
+// Host values of a run's device clock (DeviceDiag::t .. rng), and what a
+// batch reads back of it.
+struct ClockInit {
+  double t, t_final, fixed_dt, c, lam_diff_const, k2s;
+  unsigned long long rng;
+};
+struct ClockOut {
+  double t, dt;                       // dt: the last step, or a stalled clock's rule
+  long long steps;
+  unsigned long long rng;
+  unsigned int flags;
+};
+
+// Set a run's device clock (a kernel, not a copy, so it needs no staging
+// memory); clear: also the NaN flag and the rates (a fresh served slot).
+__global__ void set_clock_kernel(DeviceDiag *d, const ClockInit k, const int clear)
+{
+  if(clear) { d->flags = 0; d->pad = 0; d->lam_adv_bits = 0; d->lam_diff_bits = 0; }
+  d->flags &= ~2u;                    // no stalled clock
+  d->t = k.t; d->t_final = k.t_final; d->dt = 0.0;
+  d->fixed_dt = k.fixed_dt; d->c = k.c; d->lam_diff_const = k.lam_diff_const; d->k2s = k.k2s;
+  d->rng = k.rng;
+  d->steps = 0;
+}
+
 } // namespace
 
 struct fg2_run {
@@ -183,9 +208,9 @@ int tile_rows(long n1, long nstrips, long slots, long h_min, long extra)
 
 // Persistent grid of a batched launch: one resident round of workgroups of
 // the batch kernel that runs (its hardware occupancy times the CUs).
-int batch_grid(int module, bool step, int waves, int cus)
+int batch_grid(int module, bool step, int waves, int cus, bool cfl = false)
 {
-  return std::max(1, blocks_per_cu(module, step ? 3 : 2, waves) * cus);
+  return std::max(1, blocks_per_cu(module, step ? (cfl ? 5 : 3) : 2, waves) * cus);
 }
 
 // Substage kernels: a tile is one wavefront's 58-column strip, loading 6
@@ -335,6 +360,21 @@ double dt_from_rates(const fg2_run *r, double lam_adv, double lam_diff_cell)
   if(lam_adv > 0) dt = std::min(dt, RK3_IMAG / lam_adv);
   if(lam_diff > 0) dt = std::min(dt, RK3_REAL / lam_diff);
   return r->c * dt;
+}
+
+// A run's device clock from t to t_final: a fixed step (fixed_dt > 0) or the
+// CFL rule of dt_from_rates, with the run's forcing stream.
+ClockInit clock_of(const fg2_run *r, double t, double t_final, double fixed_dt)
+{
+  ClockInit k;
+  k.t = t;
+  k.t_final = t_final;
+  k.fixed_dt = fixed_dt;
+  k.c = r->c;
+  k.lam_diff_const = r->module == DISK ? -1.0 : lam_diff_const(r);
+  k.k2s = KMAX2 * (1.0 / ((double)r->Delta1 * r->Delta1) + 1.0 / ((double)r->Delta2 * r->Delta2));
+  k.rng = r->rng;
+  return k;
 }
 
 double bits_to_double(unsigned long long b)
@@ -1082,12 +1122,19 @@ struct fg2_batch {
   int *counters = nullptr;   // one per stage
   hipStream_t stream = nullptr;
   int parity = 0;
-  // hipGraph of the last fg2_batch_graph_run, kept for replays while its
-  // launch shape (steps, waves, grid, tiles) is unchanged; the launch records
-  // it reads are refilled before every call
+  int fo_blocks = 0;         // blocks of the batched Euler pre-step (0: no run has one)
+  bool clocked = false;      // each step takes its dt from the runs' device clocks
+  bool any_cfl = false;      // some record reduces CFL rates (clocked CFL runs)
+  // hipGraph of the last fg2_batch_graph_run / fg2_batch_solve, kept for
+  // replays while its launch shape (steps, waves, grid, tiles, pre-steps,
+  // clock) is unchanged; the launch records it reads are refilled before
+  // every call
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  int g_steps = -1, g_waves = 0, g_grid = 0, g_total = 0;
+  int g_steps = -1, g_waves = 0, g_grid = 0, g_total = 0, g_fo = 0;
+  bool g_clocked = false, g_cfl = false;
+  // fg2_batch_solve: each run's clock, gathered after every replay
+  ClockOut *clk_dev = nullptr, *clk_host = nullptr;
 };
 
 namespace {
@@ -1097,88 +1144,165 @@ __global__ void zero_ints_kernel(int *p, int n)
   if((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
-int batch_fill_step(fg2_batch *b, const double *dts)
+__global__ void gather_clock_kernel(const BatchRun *set0, const int nruns, ClockOut *out)
 {
-  for(int par = 0; par < 2; ++par) {
-    auto &H = b->host[0][par];
-    H.resize(b->nruns);
-    int begin = 0;
-    for(int i = 0; i < b->nruns; ++i) {
-      fg2_run *r = b->runs[i];
-      const int cur = r->cur ^ par;
-      BatchRun &R = H[i];
-      R = BatchRun{};
-      R.uin = r->in[cur];
-      R.uout = r->in[cur ^ 1];
-      R.v = r->in[2];
-      R.s = r->s;
-      R.P = make_params(r);
-      R.tile_h = b->tile_h[i];
-      R.ntiles = (int)step_launch_tiles(r->n1, r->n2, R.tile_h, b->waves, &R.nstrips);
-      R.tile_begin = begin;
-      begin += R.ntiles;
-      for(int k = 0; k < 3; ++k) R.g[k] = (float)(dts[i] * RK_GAMMA[k]);
-      R.diag = r->diag;
-    }
-    b->total = begin;
-    HIP_TRY(hipMemcpyAsync(b->dev[0][par], H.data(), b->nruns * sizeof(BatchRun),
-                           hipMemcpyHostToDevice, b->stream));
-  }
-  return FG2_OK;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= nruns) return;
+  const DeviceDiag *d = set0[i].diag;
+  out[i] = ClockOut{d->t, d->dt, d->steps, d->rng, d->flags};
 }
 
-// Launch records of one batched step of every run.  timing: records for the
-// autotuner that leave every run's state unchanged -- each substage reads
-// the state u[cur] and writes only the other state buffer and v (which is
-// dead at the start of a step); the one-pass step already writes only the
-// other state buffer.
+// One run's launch record: the one-pass step (stage < 0) or substage
+// `stage`, reading the state buffer in[ub] and writing in[ub ^ 1] (and v).
+// dt: the host's step (NULL: the device clock writes it before each launch,
+// and then a run in CFL mode also reduces its new state's rates).  The
+// caller sets tile_begin and fo_begin.
+BatchRun make_record(fg2_run *r, int ub, int stage, int tile_h, int waves, const double *dt,
+                     bool clocked)
+{
+  BatchRun R{};
+  R.uin = r->in[ub];
+  R.uout = r->in[ub ^ 1];
+  R.v = r->in[2];
+  R.s = r->s;
+  R.P = make_params(r);
+  R.tile_h = tile_h;
+  if(stage < 0) {
+    R.ntiles = (int)step_launch_tiles(r->n1, r->n2, tile_h, waves, &R.nstrips);
+    for(int k = 0; k < 3; ++k) R.g[k] = dt ? (float)(*dt * RK_GAMMA[k]) : 0.0f;
+  } else {
+    R.nstrips = (r->n2 + STRIP - 1) / STRIP;
+    R.ntiles = R.nstrips * ((r->n1 + tile_h - 1) / tile_h);
+    R.beta = (float)RK_BETA[stage];
+    R.gdt = dt ? (float)(*dt * RK_GAMMA[stage]) : 0.0f;
+  }
+  R.cfl = clocked && r->dt_mode == FG2_DT_CFL;
+  R.fo = first_order_active(r);
+  R.edt = dt ? (float)*dt : 0.0f;
+  R.diag = r->diag;
+  return R;
+}
+
+// Running sums of the records' tiles and Euler blocks; returns the tiles,
+// *fo_blocks the Euler blocks.
+int number_records(BatchRun *R, int n, int *fo_blocks)
+{
+  int tiles = 0, blocks = 0;
+  for(int i = 0; i < n; ++i) {
+    R[i].tile_begin = tiles;
+    tiles += R[i].ntiles;
+    R[i].fo_begin = blocks;
+    if(R[i].fo) blocks += (int)(((long)R[i].P.n1 * R[i].P.n2 + 255) / 256);
+  }
+  if(fo_blocks) *fo_blocks = blocks;
+  return tiles;
+}
+
+// Launch records of one batched step of every run, for both parities of the
+// state buffers (parity p: the step starting from in[cur ^ p]).  timing:
+// records for the autotuner that leave every run's state unchanged -- each
+// substage reads the state u[cur] and writes only the other state buffer
+// and v (which is dead at the start of a step); the one-pass step already
+// writes only the other state buffer.  dts = NULL: the device clock.
 int batch_fill(fg2_batch *b, const double *dts, bool timing = false)
 {
-  if(b->step) return batch_fill_step(b, dts);
-  // stage k reads u[cur ^ (k & 1)] and writes u[cur ^ 1 ^ (k & 1)] of each
-  // run, where cur is the run's state buffer at the start of the step
+  const int nstage = b->step ? 1 : 3;
   for(int par = 0; par < 2; ++par)
-    for(int k = 0; k < 3; ++k) {
+    for(int k = 0; k < nstage; ++k) {
       auto &H = b->host[k][par];
       H.resize(b->nruns);
-      int begin = 0;
       for(int i = 0; i < b->nruns; ++i) {
         fg2_run *r = b->runs[i];
-        const int cur = r->cur ^ par;
-        const int flip = timing ? 0 : (k & 1);
-        BatchRun &R = H[i];
-        R.uin = r->in[cur ^ flip];
-        R.uout = r->in[cur ^ 1 ^ flip];
-        R.v = r->in[2];
-        R.s = r->s;
-        R.P = make_params(r);
-        R.tile_h = b->tile_h[i];
-        R.nstrips = (r->n2 + STRIP - 1) / STRIP;
-        R.ntiles = R.nstrips * ((r->n1 + R.tile_h - 1) / R.tile_h);
-        R.tile_begin = begin;
-        begin += R.ntiles;
-        R.beta = (float)RK_BETA[k];
-        R.gdt = (float)(dts[i] * RK_GAMMA[k]);
-        R.diag = r->diag;
+        // stage k reads u[cur ^ (k & 1)] and writes u[cur ^ 1 ^ (k & 1)]
+        const int flip = (b->step || timing) ? 0 : (k & 1);
+        H[i] = make_record(r, (r->cur ^ par) ^ flip, b->step ? -1 : k, b->tile_h[i], b->waves,
+                           dts ? dts + i : nullptr, b->clocked);
       }
-      b->total = begin;
+      b->total = number_records(H.data(), b->nruns, &b->fo_blocks);
       HIP_TRY(hipMemcpyAsync(b->dev[k][par], H.data(), b->nruns * sizeof(BatchRun),
                              hipMemcpyHostToDevice, b->stream));
     }
+  b->any_cfl = false;
+  for(const BatchRun &R : b->host[0][0]) b->any_cfl = b->any_cfl || R.cfl;
   return FG2_OK;
 }
 
-int batch_launch(fg2_batch *b, int par)
+// One batched step on the batch's stream from the records of parity par:
+// the device clock (clocked batches), the Euler pre-step of the runs that
+// have one, then the step or its three substages.  timing: the step alone.
+int batch_launch(fg2_batch *b, int par, bool timing = false)
 {
+  if(b->clocked && !timing)
+    HIP_TRY(launch_clock(b->module, b->dev[0][par], b->dev[1][par], b->dev[2][par],
+                         b->step ? 1 : 3, b->step, b->nruns, b->stream));
+  if(b->fo_blocks && !timing)
+    HIP_TRY(launch_batch_euler(b->module, b->dev[0][par], b->nruns, b->fo_blocks, b->stream));
   zero_ints_kernel<<<1, 64, 0, b->stream>>>(b->counters, 3);
   if(b->step) {
+    // the CFL variant of the kernel has its own occupancy, so its own grid
+    const bool cfl = b->any_cfl && !timing;
     HIP_TRY(launch_batch_step(b->module, b->dev[0][par], b->nruns, b->total, b->waves,
-                              b->grid, b->counters, b->stream));
+                              cfl ? batch_grid(b->module, true, b->waves, b->cus, true) : b->grid,
+                              b->counters, cfl, b->stream));
     return FG2_OK;
   }
   for(int k = 0; k < 3; ++k)
     HIP_TRY(launch_batch_substage(b->module, k, b->dev[k][par], b->nruns, b->total, b->waves,
-                                  b->grid, b->counters + k, 0, b->stream));
+                                  b->grid, b->counters + k, b->stream));
+  return FG2_OK;
+}
+
+// Capture `steps` batched steps (parity alternating from 0) into the
+// batch's graph, or keep the one captured last when its shape is the same.
+int batch_graph(fg2_batch *b, int steps)
+{
+  const bool reuse = b->gexec && b->g_steps == steps && b->g_waves == b->waves &&
+                     b->g_grid == b->grid && b->g_total == b->total &&
+                     b->g_fo == b->fo_blocks && b->g_clocked == b->clocked &&
+                     b->g_cfl == b->any_cfl;
+  if(reuse) return FG2_OK;
+  if(b->gexec) { HIP_TRY(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
+  if(b->graph) { HIP_TRY(hipGraphDestroy(b->graph)); b->graph = nullptr; }
+  b->g_steps = -1;
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int err = FG2_OK;
+  for(int k = 0; k < steps && !err; ++k) err = batch_launch(b, k & 1);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+  if(err) { if(g) (void)hipGraphDestroy(g); return err; }
+  if(ce != hipSuccess) return fail(FG2_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  b->g_steps = steps; b->g_waves = b->waves; b->g_grid = b->grid; b->g_total = b->total;
+  b->g_fo = b->fo_blocks; b->g_clocked = b->clocked; b->g_cfl = b->any_cfl;
+  return FG2_OK;
+}
+
+// Order the batch's stream after every run's queued work (uploads).
+int batch_join_runs(fg2_batch *b)
+{
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for(fg2_run *r : b->runs) HIP_TRY(hipStreamSynchronize(r->stream));
+  return FG2_OK;
+}
+
+// Set every run's device clock (t[i] .. t_final[i], or with t = NULL a
+// fixed step dts[i] and no end) and, for runs in CFL mode, the rates of the
+// current state; on the batch's stream, which the caller then drains.
+int batch_set_clocks(fg2_batch *b, const double *t, const double *t_final, const double *dts)
+{
+  for(int i = 0; i < b->nruns; ++i) {
+    fg2_run *r = b->runs[i];
+    const bool cfl = t && r->dt_mode == FG2_DT_CFL;
+    const ClockInit k = t ? clock_of(r, t[i], t_final[i], cfl ? 0.0 : fg2_getdt(r))
+                          : clock_of(r, 0.0, INFINITY, dts[i]);
+    set_clock_kernel<<<1, 1, 0, b->stream>>>(r->diag, k, 0);
+    if(cfl && !r->lam_valid) {
+      reset_diag_kernel<<<1, 1, 0, b->stream>>>(r->diag, 1);
+      HIP_TRY(launch_cfl(r->module, r->in[r->cur], r->s, make_params(r), r->diag, b->stream));
+    }
+  }
+  HIP_TRY(hipGetLastError());
   return FG2_OK;
 }
 
@@ -1230,6 +1354,8 @@ void fg2_batch_free(fg2_batch *b)
   for(int k = 0; k < 3; ++k)
     for(int p = 0; p < 2; ++p) if(b->dev[k][p]) (void)hipFree(b->dev[k][p]);
   if(b->counters) (void)hipFree(b->counters);
+  if(b->clk_dev) (void)hipFree(b->clk_dev);
+  if(b->clk_host) (void)hipHostFree(b->clk_host);
   if(b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if(b->graph) (void)hipGraphDestroy(b->graph);
   if(b->stream) (void)hipStreamDestroy(b->stream);
@@ -1252,6 +1378,7 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
     flags.push_back(r->host_diag->flags);
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
+  b->clocked = false;
   // A candidate is a tile height h and a width; each is timed twice: every
   // run cut into h-row tiles (the last one shorter), and every run cut into
   // the same number of tiles of equal height (at most h rows), which keeps
@@ -1300,11 +1427,11 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
     for(int i = 0; i < b->nruns; ++i) b->tile_h[i] = height(c.tile_h, even, i);
     HIP_TRY(hipStreamSynchronize(b->stream));
     if(int e = batch_fill(b, dts, true)) return e;
-    if(int e = batch_launch(b, 0)) return e;
+    if(int e = batch_launch(b, 0, true)) return e;
     float ms = 1e30f;                       // the least of three timed launches
     for(int rep = 0; rep < 3; ++rep) {
       HIP_TRY(hipEventRecord(e0, b->stream));
-      if(int e = batch_launch(b, 0)) return e;
+      if(int e = batch_launch(b, 0, true)) return e;
       HIP_TRY(hipEventRecord(e1, b->stream));
       HIP_TRY(hipEventSynchronize(e1));
       float t = 0;
@@ -1331,15 +1458,11 @@ int fg2_batch_autotune(fg2_batch *b, const double *dts, int *tile_h, int *waves)
 int fg2_batch_step_async(fg2_batch *b, const double *dts)
 {
   if(!b || !dts) return fail(FG2_ERR_ARG, "NULL argument");
-  // the host copies of the launch records are rewritten below
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  // the runs' own streams may still hold work (uploads): order after them
-  for(fg2_run *r : b->runs) {
-    HIP_TRY(hipStreamSynchronize(r->stream));
-    randomize(r);
-    if(first_order_active(r))
-      return fail(FG2_ERR_ARG, "batched runs with an active first_order are not supported");
-  }
+  // the host copies of the launch records are rewritten below, after the
+  // runs' own queued work (uploads)
+  if(int e = batch_join_runs(b)) return e;
+  for(fg2_run *r : b->runs) randomize(r);
+  b->clocked = false;
   if(int e = batch_fill(b, dts)) return e;
   if(int e = batch_launch(b, 0)) return e;
   for(fg2_run *r : b->runs) { r->cur ^= 1; r->lam_valid = false; }
@@ -1361,31 +1484,22 @@ int fg2_batch_step(fg2_batch *b, const double *dts)
 int fg2_batch_graph_run(fg2_batch *b, const double *dts, int steps, int replays)
 {
   if(!b || !dts || steps < 1 || replays < 0) return fail(FG2_ERR_ARG, "bad graph arguments");
-  for(fg2_run *r : b->runs) {
-    HIP_TRY(hipStreamSynchronize(r->stream));
-    if(first_order_active(r))
-      return fail(FG2_ERR_ARG, "batched runs with an active first_order are not supported");
-  }
   if((steps & 1) && replays > 1)
     return fail(FG2_ERR_ARG, "replaying an odd step count more than once is not supported");
+  if(int e = batch_join_runs(b)) return e;
+  // runs with a hydro-isot forcing draw new phases every step: the device
+  // clock draws them (from each run's own stream) with the fixed dts
+  bool forced = false;
+  for(fg2_run *r : b->runs) forced = forced || (r->module == HYDRO_ISOT && first_order_active(r));
+  b->clocked = forced;
+  if(forced) {
+    // the clock draws the phases of every step of the call; the host's
+    // stream steps past them below
+    if(int e = batch_set_clocks(b, nullptr, nullptr, dts)) return e;
+  }
   if(int e = batch_fill(b, dts)) return e;
   HIP_TRY(hipStreamSynchronize(b->stream));
-  const bool reuse = b->gexec && b->g_steps == steps && b->g_waves == b->waves &&
-                     b->g_grid == b->grid && b->g_total == b->total;
-  if(!reuse) {
-    if(b->gexec) { HIP_TRY(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
-    if(b->graph) { HIP_TRY(hipGraphDestroy(b->graph)); b->graph = nullptr; }
-    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int err = FG2_OK;
-    for(int k = 0; k < steps && !err; ++k) err = batch_launch(b, k & 1);
-    hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(b->stream, &g);
-    if(err) { if(g) (void)hipGraphDestroy(g); return err; }
-    if(ce != hipSuccess) return fail(FG2_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
-    b->graph = g;
-    HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-    b->g_steps = steps; b->g_waves = b->waves; b->g_grid = b->grid; b->g_total = b->total;
-  }
+  if(int e = batch_graph(b, steps)) return e;
   for(int k = 0; k < replays; ++k) HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   for(fg2_run *r : b->runs) {
@@ -1394,7 +1508,77 @@ int fg2_batch_graph_run(fg2_batch *b, const double *dts, int steps, int replays)
     for(long k = 0; k < (long)steps * replays; ++k) randomize(r);
     r->lam_valid = false;
   }
+  b->clocked = false;
   return FG2_OK;
+}
+
+int fg2_batch_solve(fg2_batch *b, double *t, const double *t_final, int max_steps, int *steps)
+{
+  if(!b || !t || !t_final || max_steps < 1) return fail(FG2_ERR_ARG, "bad solve arguments");
+  for(int i = 0; i < b->nruns; ++i)
+    if(!(t[i] <= t_final[i])) return fail(FG2_ERR_ARG, "run %d: t = %g past t_final = %g", i, t[i], t_final[i]);
+  if(int e = batch_join_runs(b)) return e;
+  if(!b->clk_dev) {
+    HIP_TRY(hipMalloc((void **)&b->clk_dev, b->nruns * sizeof(ClockOut)));
+    HIP_TRY(hipHostMalloc((void **)&b->clk_host, b->nruns * sizeof(ClockOut), hipHostMallocDefault));
+  }
+  b->clocked = true;
+  if(int e = batch_set_clocks(b, t, t_final, nullptr)) return e;
+  if(int e = batch_fill(b, nullptr)) return e;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  // whole graphs of CHUNK steps (even: every replay ends on the records of
+  // parity 0), then single launches for the remainder of max_steps; the
+  // clocks are read back after each replay
+  constexpr int CHUNK = 16;
+  int done = 0, status = FG2_OK;
+  auto gather = [&]() -> int {
+    gather_clock_kernel<<<(b->nruns + 255) / 256, 256, 0, b->stream>>>(b->dev[0][0], b->nruns, b->clk_dev);
+    HIP_TRY(hipMemcpyAsync(b->clk_host, b->clk_dev, b->nruns * sizeof(ClockOut),
+                           hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return FG2_OK;
+  };
+  // a run is through at its end time, with a NaN, or with a stalled clock
+  auto finished = [&]() {
+    for(int i = 0; i < b->nruns; ++i)
+      if(b->clk_host[i].t < t_final[i] && !(b->clk_host[i].flags & 3u)) return false;
+    return true;
+  };
+  bool all = false;
+  if(max_steps >= CHUNK) {
+    if(int e = batch_graph(b, CHUNK)) { status = e; goto out; }
+    while(done + CHUNK <= max_steps) {
+      HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+      done += CHUNK;
+      if(int e = gather()) { status = e; goto out; }
+      if((all = finished())) break;
+    }
+  }
+  for(int par = 0; !all && done < max_steps; ++done, par ^= 1)
+    if(int e = batch_launch(b, par)) { status = e; goto out; }
+  if(int e = gather()) { status = e; goto out; }
+  // an odd count of single launches leaves the state in the other buffer
+  if((done % CHUNK) & 1)
+    for(fg2_run *r : b->runs) r->cur ^= 1;
+  for(int i = 0; i < b->nruns; ++i) {
+    fg2_run *r = b->runs[i];
+    t[i] = b->clk_host[i].t;
+    if(steps) steps[i] = (int)b->clk_host[i].steps;
+    r->rng = b->clk_host[i].rng;            // past the phases of the steps taken
+    r->lam_valid = r->dt_mode == FG2_DT_CFL;  // the last step's epilogue reduced them
+  }
+  for(fg2_run *r : b->runs) {
+    const int e = fg2_check(r);
+    if(e && !status) status = e;
+  }
+  for(int i = 0; i < b->nruns && !status; ++i)
+    if(b->clk_host[i].flags & 2u)            // as fg2_solve refuses such a step
+      status = fail(FG2_ERR_ARG, "run %d: non-positive time step %g at t = %.17g < t_final = %.17g "
+                    "after %lld steps", i, b->clk_host[i].dt, b->clk_host[i].t, t_final[i],
+                    b->clk_host[i].steps);
+out:
+  b->clocked = false;
+  return status;
 }
 
 } // extern "C"
@@ -1468,44 +1652,20 @@ extern "C" int fg2_batch_step_static(fg2_run *const *runs, int nruns, const doub
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   // order after every run's queued work (uploads), as fg2_batch_step does
   for(int i = 0; i < nruns; ++i) {
-    fg2_run *r = runs[i];
-    HIP_TRY(hipStreamSynchronize(r->stream));
-    randomize(r);
-    if(first_order_active(r))
-      return fail(FG2_ERR_ARG, "batched runs with an active first_order are not supported");
+    HIP_TRY(hipStreamSynchronize(runs[i]->stream));
+    randomize(runs[i]);
   }
   // launch records, cut as fg2_batch_create + batch_fill cut them
   const int nstage = step ? 1 : 3;
-  int total = 0;
+  int total = 0, fo_blocks = 0;
   for(int k = 0; k < nstage; ++k) {
-    int begin = 0;
     for(int i = 0; i < nruns; ++i) {
       fg2_run *r = runs[i];
-      BatchRun &R = g_static_host[k][i];
-      R = BatchRun{};
-      R.s = r->s;
-      R.P = make_params(r);
-      R.diag = r->diag;
-      R.v = r->in[2];
-      if(step) {
-        R.tile_h = std::min(tile_h > 0 ? tile_h : 64, r->n1);
-        R.uin = r->in[r->cur];
-        R.uout = r->in[r->cur ^ 1];
-        R.ntiles = (int)step_launch_tiles(r->n1, r->n2, R.tile_h, waves, &R.nstrips);
-        for(int q = 0; q < 3; ++q) R.g[q] = (float)(dts[i] * RK_GAMMA[q]);
-      } else {
-        R.tile_h = std::min(tile_h > 0 ? tile_h : shape_of(r).tile_h, r->n1);
-        R.uin = r->in[r->cur ^ (k & 1)];
-        R.uout = r->in[r->cur ^ 1 ^ (k & 1)];
-        R.nstrips = (r->n2 + STRIP - 1) / STRIP;
-        R.ntiles = R.nstrips * ((r->n1 + R.tile_h - 1) / R.tile_h);
-        R.beta = (float)RK_BETA[k];
-        R.gdt = (float)(dts[i] * RK_GAMMA[k]);
-      }
-      R.tile_begin = begin;
-      begin += R.ntiles;
+      const int h = std::min(tile_h > 0 ? tile_h : (step ? 64 : shape_of(r).tile_h), r->n1);
+      g_static_host[k][i] = make_record(r, r->cur ^ (step ? 0 : (k & 1)), step ? -1 : k, h, waves,
+                                        dts + i, false);
     }
-    total = begin;
+    total = number_records(g_static_host[k], nruns, &fo_blocks);
   }
   hipStream_t st = runs[0]->stream;
   BatchRun *rec = (BatchRun *)base;
@@ -1513,14 +1673,15 @@ extern "C" int fg2_batch_step_static(fg2_run *const *runs, int nruns, const doub
   for(int k = 0; k < nstage; ++k)
     HIP_TRY(hipMemcpyAsync(rec + (size_t)k * STATIC_MAX_RUNS, g_static_host[k],
                            nruns * sizeof(BatchRun), hipMemcpyHostToDevice, st));
+  if(fo_blocks) HIP_TRY(launch_batch_euler(module, rec, nruns, fo_blocks, st));
   zero_ints_kernel<<<1, 64, 0, st>>>(counters, 3);      // the queues start at tile 0
   const int grid = batch_grid(module, step, waves, cus);
   if(step) {
-    HIP_TRY(launch_batch_step(module, rec, nruns, total, waves, grid, counters, st));
+    HIP_TRY(launch_batch_step(module, rec, nruns, total, waves, grid, counters, 0, st));
   } else {
     for(int k = 0; k < 3; ++k)
       HIP_TRY(launch_batch_substage(module, k, rec + (size_t)k * STATIC_MAX_RUNS, nruns, total,
-                                    waves, grid, counters + k, 0, st));
+                                    waves, grid, counters + k, st));
   }
   HIP_TRY(hipStreamSynchronize(st));
   int status = FG2_OK;
@@ -1563,6 +1724,8 @@ struct Request {
   int cur = 0, diag_slot = -1;
   Params P{};
   double dt = 0;
+  bool cfl = false, fo = false;        // CFL steps; an active first_order pre-step
+  ClockInit clk{};                     // the run's device clock from admission on
   FusedShape shape{0, 4};              // per-run mode: the run's own launch shape
   int sidx = 0;                        // per-run mode: the pool stream it steps on
   int status = FG2_OK;
@@ -1693,7 +1856,7 @@ int server_admit(fg2_server *S, double now)
     q->in[1] = (float *)(S->arena + off + one) + io;
     q->diag_slot = S->free_diag.back();
     S->free_diag.pop_back();
-    HIP_TRY(hipMemsetAsync(S->diags + q->diag_slot, 0, sizeof(DeviceDiag), S->copy));
+    set_clock_kernel<<<1, 1, 0, S->copy>>>(S->diags + q->diag_slot, q->clk, 1);
     const size_t hp = (size_t)q->n2 * q->nv * sizeof(float);
     const int u = S->up_slot;
     S->up_slot = (S->up_slot + 1) % fg2_server::STAGES;
@@ -1708,6 +1871,8 @@ int server_admit(fg2_server *S, double now)
       S->next_sidx = (S->next_sidx + 1) % fg2_server::NPOOL;
     }
     HIP_TRY(hipStreamWaitEvent(run_stream(S, q), S->up_done[u], 0));  // steps after the upload
+    if(q->cfl)                           // the first step's rates: those of the uploaded state
+      HIP_TRY(launch_cfl(q->module, q->in[0], q->s, q->P, S->diags + q->diag_slot, run_stream(S, q)));
     q->t_admit = now;
     q->active = true;
     S->in_use += q->bytes;
@@ -1766,9 +1931,9 @@ int server_launch(fg2_server *S)
   if(S->mode == FG2_SERVE_PER_RUN) return server_launch_per_run(S);
   for(int m = 0; m < 2; ++m) {
     S->rec_host.clear();
-    int begin = 0;
     const FusedShape sh = server_shape(S, m);
     const int th = sh.tile_h;
+    bool clocked = false, cfl = false;
     for(Request *q : S->active) {
       if(q->module != m || q->done_steps >= q->steps) continue;
       BatchRun R{};
@@ -1778,13 +1943,19 @@ int server_launch(fg2_server *S)
       R.P = q->P;
       R.tile_h = std::min(th, q->n1);
       R.ntiles = (int)step_launch_tiles(q->n1, q->n2, R.tile_h, sh.waves, &R.nstrips);
-      R.tile_begin = begin;
-      begin += R.ntiles;
       for(int k = 0; k < 3; ++k) R.g[k] = (float)(q->dt * RK_GAMMA[k]);
+      R.cfl = q->cfl;
+      R.fo = q->fo;
+      R.edt = (float)q->dt;
       R.diag = S->diags + q->diag_slot;
       S->rec_host.push_back(R);
+      // CFL steps, and forcing phases drawn every step, come from the clock
+      clocked = clocked || q->cfl || (q->fo && m == HYDRO_ISOT);
+      cfl = cfl || q->cfl;
     }
     if(S->rec_host.empty()) continue;
+    int fo_blocks = 0;
+    const int begin = number_records(S->rec_host.data(), (int)S->rec_host.size(), &fo_blocks);
     if(int e = server_grow(S, S->rec_host.size())) return e;
     const int slot = S->rec_slot;
     S->rec_slot = (S->rec_slot + 1) % fg2_server::SLOTS;
@@ -1792,9 +1963,14 @@ int server_launch(fg2_server *S)
     memcpy(S->rec_pinned[slot], S->rec_host.data(), S->rec_host.size() * sizeof(BatchRun));
     HIP_TRY(hipMemcpyAsync(S->rec_dev[slot], S->rec_pinned[slot], S->rec_host.size() * sizeof(BatchRun),
                            hipMemcpyHostToDevice, S->stream));
+    const int n = (int)S->rec_host.size();
+    if(clocked)
+      HIP_TRY(launch_clock(m, S->rec_dev[slot], nullptr, nullptr, 1, 1, n, S->stream));
+    if(fo_blocks) HIP_TRY(launch_batch_euler(m, S->rec_dev[slot], n, fo_blocks, S->stream));
     zero_ints_kernel<<<1, 64, 0, S->stream>>>(S->counters + slot, 1);
-    HIP_TRY(launch_batch_step(m, S->rec_dev[slot], (int)S->rec_host.size(), begin, sh.waves,
-                              batch_grid(m, true, sh.waves, S->cus), S->counters + slot, S->stream));
+    HIP_TRY(launch_batch_step(m, S->rec_dev[slot], n, begin, sh.waves,
+                              batch_grid(m, true, sh.waves, S->cus, cfl), S->counters + slot,
+                              cfl, S->stream));
     HIP_TRY(hipEventRecord(S->rec_done[slot], S->stream));
   }
   for(Request *q : S->active) {
@@ -1836,7 +2012,8 @@ int server_retire(fg2_server *S, double now)
       while(S->down_owner[d] != q) ++d;
       memcpy(q->host.data(), S->down[d], q->host.size() * sizeof(float));
       S->down_owner[d] = nullptr;
-      q->status = (S->diags_host[q->diag_slot].flags & 1u) ? FG2_ERR_NAN : FG2_OK;
+      const unsigned int fl = S->diags_host[q->diag_slot].flags;
+      q->status = (fl & 1u) ? FG2_ERR_NAN : (fl & 2u) ? FG2_ERR_ARG : FG2_OK;
       q->t_done = now;
       arena_free(S, q->off, q->bytes);
       S->in_use -= q->bytes;
@@ -1940,22 +2117,45 @@ int fg2_server_submit(fg2_server *S, const char *module, int n1, int n2, int ste
   const int m = module_id(module);
   if(m != HYDRO && m != HYDRO_ISOT) return fail(FG2_ERR_ARG, "served modules: hydro, hydro-isot");
   if(n1 < HALF || n2 < HALF) return fail(FG2_ERR_SHAPE, "grid %d x %d too small", n1, n2);
+  // geometry, parameters and dt of a default run of this module (periodic,
+  // src/global.cpp:28-29 and the module's settings()), no device memory
+  fg2_run tmp;
+  tmp.module = m; tmp.nv = nvar_of(m); tmp.n1 = n1; tmp.n2 = n2; tmp.cus = S->cus;
+  for(auto &p : param_defs(m)) tmp.par.push_back(p.value);
+  fg2_config(&tmp);
+  return fg2_server_submit_run(S, &tmp, steps, init, now, id);
+}
+
+int fg2_server_submit_run(fg2_server *S, const fg2_run *like, int steps, const float *init,
+                          double now, long long *id)
+{
+  if(!S || !like || !init || steps < 1) return fail(FG2_ERR_ARG, "bad request");
+  if(int e = check_run(like)) return e;
+  const int m = like->module;
+  if(m != HYDRO && m != HYDRO_ISOT) return fail(FG2_ERR_ARG, "served modules: hydro, hydro-isot");
+  if(!like->p1 || !like->p2) return fail(FG2_ERR_ARG, "served runs are periodic");
+  if(like->slab_n1g) return fail(FG2_ERR_ARG, "served runs are whole grids, not slabs");
+  const int n1 = like->n1, n2 = like->n2;
   const size_t need = 2 * (size_t)array_floats(n1, n2, nvar_of(m)) * sizeof(float);
   if(need > S->budget) return fail(FG2_ERR_NOMEM, "request needs %zu bytes, budget %zu", need, S->budget);
+  const bool cfl = like->dt_mode == FG2_DT_CFL, fo = first_order_active(like);
+  if(S->mode == FG2_SERVE_PER_RUN && (cfl || fo))
+    return fail(FG2_ERR_ARG, "per-run mode serves fixed-dt runs without a first_order pre-step");
   Request *q = new(std::nothrow) Request;
   if(!q) return fail(FG2_ERR_NOMEM, "host allocation failed");
   q->id = S->next_id++;
   q->module = m; q->n1 = n1; q->n2 = n2; q->nv = nvar_of(m); q->steps = steps;
   q->host.assign(init, init + (size_t)n1 * n2 * q->nv);
-  // geometry, parameters and dt of a default run of this module (periodic,
-  // src/global.cpp:28-29 and the module's settings()), no device memory
-  fg2_run tmp;
-  tmp.module = m; tmp.nv = q->nv; tmp.n1 = n1; tmp.n2 = n2; tmp.cus = S->cus;
-  for(auto &p : param_defs(m)) tmp.par.push_back(p.value);
-  fg2_config(&tmp);
-  q->P = make_params(&tmp);
-  q->dt = fg2_getdt(&tmp);
-  q->shape = step_shape_of(&tmp);
+  // the run's parameters, geometry, dt rule and forcing stream
+  q->P = make_params(like);
+  q->cfl = cfl;
+  q->fo = fo;
+  q->dt = fg2_getdt(like);
+  q->clk = clock_of(like, 0.0, INFINITY, cfl ? 0.0 : q->dt);
+  fg2_run shape_run;
+  shape_run.module = m; shape_run.nv = q->nv; shape_run.n1 = n1; shape_run.n2 = n2;
+  shape_run.cus = S->cus;
+  q->shape = step_shape_of(&shape_run);
   q->t_arrive = now;
   S->queue.push_back(q);
   S->all[q->id] = q;

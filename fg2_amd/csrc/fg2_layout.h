@@ -81,12 +81,25 @@ struct SlabHalo {
 
 // Diagnostics written by the kernels.  flags bit 0: a NaN was produced
 // (the sticky contain_nan of src/drift.cu:21,30).  An Inf is not flagged: as
-// in the reference, only a NaN fails a step (src/step.cpp:38-40).
+// in the reference, only a NaN fails a step (src/step.cpp:38-40).  Bit 1:
+// the device clock's dt rule gave no positive step before t_final.
 struct DeviceDiag {
   unsigned int flags;
   unsigned int pad;
   unsigned long long lam_adv_bits;   // max advective rate, FP64 bits (>= 0)
   unsigned long long lam_diff_bits;  // max diffusive rate, FP64 bits (>= 0)
+  // Clock of a run whose steps are queued without the host (batches,
+  // hipGraph replays, served runs).  Before each such step the clock update
+  // takes dt = min(t_final - t, rule), the rule being fixed_dt when it is
+  // > 0 and otherwise c min(sqrt3 / lam_adv, 2.5127 / lam_diff) from the
+  // rates above (lam_diff = lam_diff_const, or when that is < 0 the per-cell
+  // rate times k2s), advances t, draws the hydro-isot forcing phases from
+  // rng, and clears the rates for the step's epilogue.  dt = 0 once t has
+  // reached t_final: the run then stands still.
+  double t, t_final, dt;
+  double fixed_dt, c, lam_diff_const, k2s;
+  unsigned long long rng;            // SplitMix64 state of the forcing phases
+  long long steps;                   // steps of dt > 0 taken
 };
 
 } // namespace fg2

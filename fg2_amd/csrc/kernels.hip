@@ -622,8 +622,11 @@ void bounded_step_kernel(const float *__restrict__ uin, float *__restrict__ uout
 
 // Batched one-pass step: a persistent grid of workgroups pulls step tiles of
 // many periodic runs from one atomic counter (same step_tile as above, so a
-// run gives the same bits alone and in a batch).
-template<class M, int NW>
+// run gives the same bits alone and in a batch).  CFL: the variant that
+// reduces the rates of the runs whose records ask for it (the reduction
+// costs the hydro-isot kernel its third wavefront per SIMD, so batches
+// without CFL runs take the variant without it).
+template<class M, int NW, bool CFL>
 __global__ __launch_bounds__(64 * NW)
 void batch_step_kernel(const BatchRun *__restrict__ runs, const int nruns, const int total_tiles,
                        int *counter)
@@ -646,26 +649,37 @@ void batch_step_kernel(const BatchRun *__restrict__ runs, const int nruns, const
     // of 64 NW lanes, then a packed narrow strip)
     const StepTileShape T = step_tile_shape<NW>(t - R.tile_begin, R.P.n1, R.tile_h, R.nstrips);
     TileOut res{0, 0.0f, 0.0f};
+    const int cfl = CFL ? R.cfl : 0;
     step_tile<M, NW>(R.uin, R.uout, R.s, R.P, R.g[0], R.g[1], R.g[2], (float)(-5.0 / 9.0),
-                     (float)(-153.0 / 128.0), T.j0, T.i0, T.i1, T.lane0, T.wl, T.nit, 0, L, res);
-    commit_tile(res, 0, R.diag);
+                     (float)(-153.0 / 128.0), T.j0, T.i0, T.i1, T.lane0, T.wl, T.nit, cfl, L,
+                     res);
+    commit_tile(res, cfl, R.diag);
   }
 }
 
-// Batched substage: a persistent grid of wavefronts pulls tiles of many runs
-// from one atomic counter.  Every tile runs the same process_tile as the
-// single-run kernel, so a run gives the same bits alone and in a batch.
+// Batched substage: a persistent grid of workgroups pulls tiles of many runs
+// from one atomic counter, WPB consecutive tiles per workgroup (one per
+// wavefront) behind a barrier.  Every tile runs the same process_tile as
+// the single-run kernel, so a run gives the same bits alone and in a batch.
+// The fetch is per workgroup, behind barriers, because a per-wavefront
+// fetch (lane 0's atomic broadcast by a lane shuffle) compiled to a loop
+// whose lanes 1..63 went round again without lane 0 after a tile that
+// reduces CFL rates, reading a stale tile index for ever.
 template<class M, int STAGE, int WPB>
 __global__ __launch_bounds__(64 * WPB)
 void batch_kernel(const BatchRun *__restrict__ runs, const int nruns, const int total_tiles,
-                  int *counter, const int cfl)
+                  int *counter)
 {
-  const int lane = threadIdx.x & 63;
+  __shared__ int next;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   for(;;) {
-    int t = 0;
-    if(lane == 0) t = atomicAdd(counter, 1);
-    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
-    if(t >= total_tiles) break;
+    __syncthreads();                       // every wavefront is past the previous fetch
+    if(threadIdx.x == 0) next = atomicAdd(counter, WPB);
+    __syncthreads();
+    const int t0 = __builtin_amdgcn_readfirstlane(next);
+    if(t0 >= total_tiles) break;
+    const int t = t0 + w;
+    if(t >= total_tiles) continue;         // past the last tile: only the barriers
     // binary search for the run owning tile t (tiles are ordered by run)
     int lo = 0, hi = nruns - 1;
     while(lo < hi) {
@@ -678,6 +692,7 @@ void batch_kernel(const BatchRun *__restrict__ runs, const int nruns, const int 
     const int strip = lt % R.nstrips, chunk = lt / R.nstrips;
     const int i0 = chunk * R.tile_h, i1 = min(R.P.n1, i0 + R.tile_h);
     TileOut res{0, 0.0f, 0.0f};
+    const int cfl = STAGE == 2 ? R.cfl : 0;
     process_tile<M, STAGE>(R.uin, R.uout, R.v, R.s, R.P, R.beta, R.gdt, strip * STRIP, i0, i1,
                            cfl, res);
     commit_tile(res, cfl, R.diag);
@@ -1058,7 +1073,8 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
 // register and LDS use): kind 0 = the substage kernels (the least of the
 // three stages), kind 1 = the one-pass step kernel, kind 2 = the batched
 // substage kernels, kind 3 = the batched step kernel, kind 4 = the one-pass
-// step kernel of bounded runs.
+// step kernel of bounded runs, kind 5 = the batched step kernel with the
+// CFL reduction.
 hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
 {
   *blocks_per_cu = 0;
@@ -1098,10 +1114,14 @@ hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
           if(waves == 1) take((const void *)bounded_step_kernel<M, 1>);
           if(waves == 2) take((const void *)bounded_step_kernel<M, 2>);
           if(waves == 4) take((const void *)bounded_step_kernel<M, 4>);
+        } else if(kind == 3) {
+          if(waves == 1) take((const void *)batch_step_kernel<M, 1, false>);
+          if(waves == 2) take((const void *)batch_step_kernel<M, 2, false>);
+          if(waves == 4) take((const void *)batch_step_kernel<M, 4, false>);
         } else {
-          if(waves == 1) take((const void *)batch_step_kernel<M, 1>);
-          if(waves == 2) take((const void *)batch_step_kernel<M, 2>);
-          if(waves == 4) take((const void *)batch_step_kernel<M, 4>);
+          if(waves == 1) take((const void *)batch_step_kernel<M, 1, true>);
+          if(waves == 2) take((const void *)batch_step_kernel<M, 2, true>);
+          if(waves == 4) take((const void *)batch_step_kernel<M, 4, true>);
         }
       }
     }
@@ -1111,7 +1131,7 @@ hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
 }
 
 hipError_t launch_batch_step(int module, const BatchRun *runs_dev, int nruns, int total_tiles,
-                             int waves, int grid, int *counter, hipStream_t st)
+                             int waves, int grid, int *counter, int cfl, hipStream_t st)
 {
   return by_module(module, [&](auto m) {
     using M = decltype(m);
@@ -1121,7 +1141,12 @@ hipError_t launch_batch_step(int module, const BatchRun *runs_dev, int nruns, in
       switch(waves) {
 #define FG2_S(NW)                                                                           \
       case NW:                                                                              \
-        batch_step_kernel<M, NW><<<grid, 64 * NW, 0, st>>>(runs_dev, nruns, total_tiles, counter); \
+        if(cfl)                                                                             \
+          batch_step_kernel<M, NW, true><<<grid, 64 * NW, 0, st>>>(runs_dev, nruns,         \
+                                                                   total_tiles, counter);  \
+        else                                                                                \
+          batch_step_kernel<M, NW, false><<<grid, 64 * NW, 0, st>>>(runs_dev, nruns,        \
+                                                                    total_tiles, counter); \
         break;
       FG2_S(1) FG2_S(2) FG2_S(4)
 #undef FG2_S
@@ -1145,12 +1170,12 @@ hipError_t launch_sums(int module, const float *u, long s, const Params &P, doub
 
 template<class M, int STAGE>
 static hipError_t launch_batch_stage(const BatchRun *runs, int nruns, int total, int waves,
-                                     int grid, int *counter, int cfl, hipStream_t st)
+                                     int grid, int *counter, hipStream_t st)
 {
   switch(waves) {
 #define FG2_W(WPB)                                                                 \
   case WPB:                                                                       \
-    batch_kernel<M, STAGE, WPB><<<grid, 64 * WPB, 0, st>>>(runs, nruns, total, counter, cfl); \
+    batch_kernel<M, STAGE, WPB><<<grid, 64 * WPB, 0, st>>>(runs, nruns, total, counter); \
     break;
   FG2_W(1) FG2_W(2) FG2_W(4)
 #undef FG2_W
@@ -1160,17 +1185,134 @@ static hipError_t launch_batch_stage(const BatchRun *runs, int nruns, int total,
 }
 
 hipError_t launch_batch_substage(int module, int stage, const BatchRun *runs, int nruns, int total,
-                                 int waves, int grid, int *counter, int cfl, hipStream_t st)
+                                 int waves, int grid, int *counter, hipStream_t st)
 {
   return by_module(module, [&](auto m) {
     using M = decltype(m);
     switch(stage) {
-    case 0: return launch_batch_stage<M, 0>(runs, nruns, total, waves, grid, counter, cfl, st);
-    case 1: return launch_batch_stage<M, 1>(runs, nruns, total, waves, grid, counter, cfl, st);
-    case 2: return launch_batch_stage<M, 2>(runs, nruns, total, waves, grid, counter, cfl, st);
+    case 0: return launch_batch_stage<M, 0>(runs, nruns, total, waves, grid, counter, st);
+    case 1: return launch_batch_stage<M, 1>(runs, nruns, total, waves, grid, counter, st);
+    case 2: return launch_batch_stage<M, 2>(runs, nruns, total, waves, grid, counter, st);
     }
     return hipErrorInvalidValue;
   });
+}
+
+// ---------------------------------------------------------------------------
+// Batched Euler pre-step and the device clock
+// ---------------------------------------------------------------------------
+
+// first_order of every cell of the records with an active pre-step: block b
+// belongs to the last record whose fo_begin <= b (fo_begin is the running
+// sum of the blocks of the records before, so of equal values the last is
+// the one with blocks), and each thread takes one cell, as euler_kernel
+// does, with the same per-cell arithmetic.
+template<class M>
+__global__ __launch_bounds__(256)
+void batch_euler_kernel(const BatchRun *__restrict__ runs, const int nruns)
+{
+  const int b = blockIdx.x;
+  int lo = 0, hi = nruns - 1;
+  while(lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if(runs[mid].fo_begin <= b) lo = mid; else hi = mid - 1;
+  }
+  const BatchRun &R = runs[lo];
+  if(!R.fo || !(R.edt > 0.0f)) return;     // no pre-step, or a run standing still
+  const long cell = (long)(b - R.fo_begin) * 256 + threadIdx.x;
+  const long n = (long)R.P.n1 * R.P.n2;
+  if(cell >= n) return;
+  const int i = (int)(cell / R.P.n2), j = (int)(cell % R.P.n2);
+  float *c = const_cast<float *>(R.uin) + (long)i * R.s + (long)j * M::NV;
+  M::first_order(R.P, c, R.edt, i, j);
+}
+
+hipError_t launch_batch_euler(int module, const BatchRun *runs_dev, int nruns, int blocks,
+                              hipStream_t st)
+{
+  if(blocks < 1) return hipSuccess;
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    if constexpr(!M::HAS_FIRST_ORDER) return hipSuccess;
+    batch_euler_kernel<M><<<blocks, 256, 0, st>>>(runs_dev, nruns);
+    return hipGetLastError();
+  });
+}
+
+__device__ __forceinline__ unsigned long long splitmix64_dev(unsigned long long &state)
+{
+  unsigned long long z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// One thread per run.  The dt arithmetic is the host's dt rule (FP64,
+// IEEE division, the same order of operations), so a run takes the same
+// steps whether its dt is chosen here or by fg2_getdt_cfl.
+__global__ __launch_bounds__(256)
+void clock_kernel(BatchRun *set0, BatchRun *set1, BatchRun *set2, const int nsets,
+                  const int step, const int nruns, const int draw_phases)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= nruns) return;
+  DeviceDiag *d = set0[i].diag;
+  double rule = d->fixed_dt;
+  if(!(rule > 0.0)) {
+    constexpr double RK3_IMAG = 1.7320508075688772, RK3_REAL = 2.5127453266183286;
+    const double la = __longlong_as_double((long long)d->lam_adv_bits);
+    const double ld = d->lam_diff_const >= 0.0
+                        ? d->lam_diff_const
+                        : __longlong_as_double((long long)d->lam_diff_bits) * d->k2s;
+    double x = INFINITY;
+    if(la > 0.0) { const double y = RK3_IMAG / la; x = y < x ? y : x; }
+    if(ld > 0.0) { const double y = RK3_REAL / ld; x = y < x ? y : x; }
+    rule = d->c * x;
+  }
+  const double left = d->t_final - d->t;
+  double dt = rule < left ? rule : left;
+  float r1 = set0[i].P.rand1, r2 = set0[i].P.rand2;
+  if(dt > 0.0) {
+    const double tn = d->t + dt;
+    d->t = tn < d->t_final ? tn : d->t_final;
+    ++d->steps;
+    if(draw_phases) {                     // the host's randomize(), per step taken
+      const double u1 = (double)(splitmix64_dev(d->rng) >> 11) * 0x1.0p-53;
+      const double u2 = (double)(splitmix64_dev(d->rng) >> 11) * 0x1.0p-53;
+      r1 = (float)(6.2831853071795865 * u1);
+      r2 = (float)(6.2831853071795865 * u2);
+    }
+  } else {
+    // a rule of 0 (or less) before t_final would hold the run still for
+    // ever: flag it (DeviceDiag flags bit 1) and keep the rule for the error
+    if(left > 0.0) {
+      atomicOr(&d->flags, 2u);
+      d->dt = rule;
+    }
+    dt = 0.0;
+  }
+  if(dt > 0.0 || !(left > 0.0)) d->dt = dt;
+  d->lam_adv_bits = 0;                    // the step's epilogue reduces the new state's rates
+  d->lam_diff_bits = 0;
+  constexpr double G[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+  BatchRun *sets[3] = {set0, set1, set2};
+  for(int k = 0; k < nsets; ++k) {
+    BatchRun &R = sets[k][i];
+    R.P.rand1 = r1;
+    R.P.rand2 = r2;
+    R.edt = (float)dt;
+    if(step) { R.g[0] = (float)(dt * G[0]); R.g[1] = (float)(dt * G[1]); R.g[2] = (float)(dt * G[2]); }
+    else R.gdt = (float)(dt * G[k]);
+  }
+}
+
+hipError_t launch_clock(int module, BatchRun *set0, BatchRun *set1, BatchRun *set2, int nsets,
+                        int step, int nruns, hipStream_t st)
+{
+  if(nruns < 1) return hipSuccess;
+  clock_kernel<<<(nruns + 255) / 256, 256, 0, st>>>(set0, set1, set2, nsets, step, nruns,
+                                                     module == HYDRO_ISOT);
+  return hipGetLastError();
 }
 
 } // namespace fg2

@@ -64,8 +64,9 @@ hipError_t launch_disk_rows(float *tab, const Params &P, hipStream_t st);
 // Resident workgroups per CU (hipOccupancyMaxActiveBlocksPerMultiprocessor)
 // with `waves` wavefronts per workgroup of the substage kernels (kind 0, the
 // least over the three stages), the one-pass step kernel (kind 1), the
-// batched substage kernels (kind 2), the batched step kernel (kind 3) or the
-// one-pass step kernel of bounded runs (kind 4).
+// batched substage kernels (kind 2), the batched step kernel (kind 3), the
+// one-pass step kernel of bounded runs (kind 4) or the batched step kernel
+// with the CFL reduction (kind 5).
 hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu);
 
 // CFL rates (FP64 max over cells of the advective and diffusive rates)
@@ -110,14 +111,29 @@ struct BatchRun {
   int tile_h, ntiles, nstrips, tile_begin;  // this run's tiles are [tile_begin, tile_begin+ntiles)
   float beta, gdt;       // substage batches: this stage's beta and gamma dt
   float g[3];            // step batches: gamma_k dt of the three stages
+  int cfl;               // reduce the CFL rates of the step's new state into diag
+  int fo, fo_begin;      // Euler pre-step active; the run's first block in the batch's
+                         // Euler launch (the blocks of the records before it)
+  float edt;             // the Euler pre-step's dt
   DeviceDiag *diag;
 };
+// Batched Euler pre-step (first_order of every record with fo set; such a
+// run of n1 x n2 cells takes ceil(n1 n2 / 256) blocks of 256 threads from
+// fo_begin on; records with edt = 0 stand still).
+hipError_t launch_batch_euler(int module, const BatchRun *runs_dev, int nruns, int blocks,
+                              hipStream_t st);
+// Clock update of the runs of a batched step whose dt the device takes
+// (DeviceDiag's clock): one thread per run; writes dt into the nsets record
+// sets (stage k of set k for substage batches, set 0 holding g[] for step
+// batches), with the forcing phases and the Euler dt.  module: the batch's.
+hipError_t launch_clock(int module, BatchRun *set0, BatchRun *set1, BatchRun *set2, int nsets,
+                        int step, int nruns, hipStream_t st);
 // Batched one-pass step (every run periodic, module hydro or hydro-isot):
 // records hold uin/uout, the tile split and g[3]; one launch per step.
 hipError_t launch_batch_step(int module, const BatchRun *runs_dev, int nruns, int total_tiles,
-                             int waves, int grid, int *counter, hipStream_t st);
+                             int waves, int grid, int *counter, int cfl, hipStream_t st);
 hipError_t launch_batch_substage(int module, int stage, const BatchRun *runs_dev, int nruns,
                                  int total_tiles, int waves, int grid, int *counter,
-                                 int cfl, hipStream_t st);
+                                 hipStream_t st);
 
 } // namespace fg2

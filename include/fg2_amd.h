@@ -257,8 +257,11 @@ int fg2_perf_model(const fg2_run *run, double *flops, double *bits, double *alg_
  * the batch by one RK3 step of its own dt (dts[k]) with persistent launches
  * for the whole batch: one (the one-pass step, when every run is periodic
  * hydro or hydro-isot) or three (one per substage: disk, or a batch with a
- * Neumann or reflecting run).  The runs keep their own state; a run gives
- * the same bits alone (fg2_step, fused path) and in a batch. */
+ * Neumann or reflecting run), after one batched launch of the Euler
+ * pre-step of the runs that have one (hydro-isot stochastic forcing, f k >
+ * 0; disk radiation equilibrium, ff != 0; src/kick.cu:128-138).  The runs
+ * keep their own state; a run gives the same bits alone (fg2_step, fused
+ * path) and in a batch. */
 int fg2_batch_create(fg2_batch **batch, fg2_run *const *runs, int nruns, int waves, int tile_h);
 void fg2_batch_free(fg2_batch *batch);
 int fg2_batch_step_async(fg2_batch *batch, const double *dts);
@@ -267,8 +270,22 @@ int fg2_batch_step(fg2_batch *batch, const double *dts);
  * times (an odd `steps` only once).  The graph is kept: a later call with
  * the same `steps`, while the batch's workgroup shape and tiles are
  * unchanged, replays it without a new capture.  The per-run launch records
- * (state buffers, dts) are refilled on every call. */
+ * (state buffers, dts) are refilled on every call.  Forced hydro-isot runs
+ * (f k > 0) draw their phases on the device. */
 int fg2_batch_graph_run(fg2_batch *batch, const double *dts, int steps, int replays);
+/* Advance run k of the batch from t[k] to t_final[k], as fg2_solve advances
+ * one run (src/solve.cpp:37-49: dt = min(t_final - t, getdt()), t = min(t +
+ * dt, t_final)), with every dt chosen on the device: each run's dt mode
+ * (fg2_set_dt_mode) -- the closed form, or the FP64 CFL rule from the rates
+ * its previous step's epilogue reduced -- so no step waits for the host.
+ * Steps go in hipGraphs of 16 (kept across calls) and the host reads the
+ * runs' times once per graph; a run that reaches its end first takes steps
+ * of dt = 0, which leave its state as it is, until all have, or until
+ * max_steps steps.  t[k] receives the time reached, steps[k] (may be NULL)
+ * the steps taken.  A run gives the same bits, times and step count here
+ * as through fg2_solve.  Returns FG2_OK or FG2_ERR_NAN; FG2_ERR_ARG for a
+ * run whose dt rule gives no positive step before its end time. */
+int fg2_batch_solve(fg2_batch *batch, double *t, const double *t_final, int max_steps, int *steps);
 void *fg2_batch_stream(fg2_batch *batch);
 /* Choose the batch's tile height and workgroup shape by timing batched
  * steps of size dts on the runs' states (states unchanged).  Each height h
@@ -280,7 +297,7 @@ int fg2_batch_autotune(fg2_batch *batch, const double *dts, int *tile_h, int *wa
  * batch made by fg2_batch_create(runs, nruns, waves, tile_h) -- same
  * kernels, same tiles, same bits -- that allocates nothing: no hipMalloc,
  * hipFree or other allocating call, on any call.  The launch records and
- * tile counters live in a fixed device array of the library (about 384 KiB
+ * tile counters live in a fixed device array of the library (about 400 KiB
  * per GPU, held until the process ends), so nruns is at most
  * FG2_STATIC_MAX_RUNS (more is FG2_ERR_SHAPE).  The runs' states stay in
  * their own buffers.  Runs in the current device; the launch goes on
@@ -301,8 +318,9 @@ typedef struct fg2_server fg2_server;
 /* A server that keeps admitted runs' state on the current device in an
  * arena of hbm_budget bytes (allocated once; a run holds two state arrays)
  * and advances all active runs together each tick (one batched one-pass
- * launch per module; served runs are periodic hydro or hydro-isot with the
- * module's default parameters).  waves, tile_h: the batched kernel's
+ * launch per module, after the batched Euler pre-step of the runs that have
+ * one; served runs are periodic hydro or hydro-isot).  waves, tile_h: the
+ * batched kernel's
  * wavefronts per workgroup and tile height; 0 chooses it per launch from
  * the active runs (rounds of resident workgroups times the tile's cost). */
 int fg2_server_create(fg2_server **server, size_t hbm_budget, int waves, int tile_h);
@@ -312,6 +330,15 @@ void fg2_server_free(fg2_server *server);
  * caller's clock, used for the latency record. */
 int fg2_server_submit(fg2_server *server, const char *module, int n1, int n2, int steps,
                       const float *init, double now, long long *id);
+/* Queue a request shaped like the configured run `like`: its module and
+ * grid, geometry (periodic only), parameters (fg2_para), dt mode
+ * (fg2_set_dt_mode: the closed form, or the FP64 CFL rule chosen on the
+ * device each step) and forcing stream (fg2_seed).  Only like's host-side
+ * settings are read.  The result is the same bits as `steps` fg2_step calls
+ * on a run set up like `like`, with dt = fg2_getdt or fg2_getdt_cfl before
+ * each.  Per-run mode serves only fixed-dt runs without a pre-step. */
+int fg2_server_submit_run(fg2_server *server, const fg2_run *like, int steps, const float *init,
+                          double now, long long *id);
 /* Retire finished runs, admit queued requests that fit the budget, and
  * launch one step of every active run.  Counts of active, queued and
  * finished requests are returned (pointers may be NULL). */

@@ -29,12 +29,51 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfg2_amd.so")
 
-NVAR = {"hydro": 4, "hydro-isot": 3, "disk": 6}
-FIELDS = {
-    "hydro": ("lnd", "u1", "u2", "lne"),
-    "hydro-isot": ("lnd", "u1", "u2"),
-    "disk": ("lnd", "ur", "uz", "Omg", "lne", "lner"),
-}
+
+
+class _Modules(dict):
+    """The library's physics modules (fg2_module_count, fg2_module), read on
+    first use: name -> value(nvar, fields)."""
+
+    def __init__(self, value):
+        super().__init__()
+        self._value = value
+
+    def _fill(self):
+        if not dict.__len__(self):
+            for k in range(lib.fg2_module_count()):
+                n, nv, f = ctypes.c_char_p(), ctypes.c_int(), ctypes.c_char_p()
+                _check(lib.fg2_module(k, ctypes.byref(n), ctypes.byref(nv), ctypes.byref(f)))
+                dict.__setitem__(self, n.value.decode(),
+                                 self._value(nv.value, tuple(f.value.decode().split(","))))
+
+    def __getitem__(self, k):
+        self._fill()
+        return dict.__getitem__(self, k)
+
+    def __contains__(self, k):
+        self._fill()
+        return dict.__contains__(self, k)
+
+    def __iter__(self):
+        self._fill()
+        return dict.__iter__(self)
+
+    def __len__(self):
+        self._fill()
+        return dict.__len__(self)
+
+    def keys(self):
+        self._fill()
+        return dict.keys(self)
+
+    def items(self):
+        self._fill()
+        return dict.items(self)
+
+
+NVAR = _Modules(lambda nv, fields: nv)          # module name -> NVAR
+FIELDS = _Modules(lambda nv, fields: fields)    # module name -> state variable names
 
 OK, ERR_NAN, ERR_HIP, ERR_SHAPE, ERR_ARG, ERR_NOMEM = range(6)
 PATH_FUSED, PATH_PLAIN, PATH_SUBSTAGE = 0, 1, 2
@@ -63,6 +102,9 @@ def _load():
     SP = ctypes.POINTER(ctypes.c_size_t)
     sig = {
         "fg2_last_error": (ctypes.c_char_p, []),
+        "fg2_module_count": (I, []),
+        "fg2_module": (I, [I, ctypes.POINTER(ctypes.c_char_p), IP,
+                           ctypes.POINTER(ctypes.c_char_p)]),
         "fg2_setup": (I, [PP, ctypes.c_char_p, I, I, SP]),
         "fg2_free": (None, [P]),
         "fg2_set_geometry": (I, [P, D, D, I, I, D]),

@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libfg2_amd.so")
 COPY_EXE = os.path.join(HERE, "hbm_copy")
 DRIVER_EXE = os.path.join(HERE, "fg2run")
 SOURCES = ["kernels.hip", "capi.hip", "ics.hip", "spectra.hip"]
-HEADERS = ["fg2_layout.h", "physics.h", "launch.h"]
+HEADERS = ["fg2_layout.h", "stencil.h", "physics.h", "launch.h", "modules/hydro.h",
+           "modules/hydro_isot.h", "modules/disk.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -22,6 +22,7 @@
 
 #include "fg2_amd.h"
 #include "launch.h"
+#include "physics.h"
 
 using namespace fg2;
 
@@ -48,32 +49,17 @@ int fail(int code, const char *fmt, ...)
       return fail(FG2_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));          \
   } while(0)
 
-// Module parameters and their reference defaults (src/hydro/eqns.cuh:19-24,
-// src/hydro-isot/eqns.cuh:19-25, src/disk/eqns.cuh:19-36), in the order of
-// Params::par used by physics.h.
-struct ParamDef { const char *name; double value; };
-const std::vector<ParamDef> &param_defs(int m)
-{
-  static const std::vector<ParamDef> hydro = {
-    {"gamma", 5.0 / 3.0}, {"nus", 2.0e-4}, {"nub", 0.0}, {"kappa", 5.0e-4}, {"dd", 0.0}};
-  static const std::vector<ParamDef> isot = {
-    {"dd", 0.0}, {"nus", 2.0e-4}, {"nub", 0.0}, {"cs2", 1.0}, {"f", 0.0}, {"k", -8.0}};
-  static const std::vector<ParamDef> disk = {
-    {"M", 1.0}, {"rS", 2.0}, {"rin", 3.0}, {"gamma", 5.0 / 3.0}, {"nus", 0.0}, {"nub", 0.0},
-    {"kappa", 0.0}, {"alpha", 0.01}, {"ar", 1.0e8}, {"es", 1.0}, {"ff", 0.0}, {"dd", 1.0e-4},
-    {"nu", 2.0e-4}, {"ed", 5.0e-4}, {"rd", 5.0e-4},
-    {"eps", 0.0}};   // gravity softening length: not in the reference, 0 keeps its force law
-  return m == HYDRO ? hydro : m == HYDRO_ISOT ? isot : disk;
-}
-
-int module_id(const char *name)
-{
-  if(!name) return -1;
-  if(!strcmp(name, "hydro")) return HYDRO;
-  if(!strcmp(name, "hydro-isot")) return HYDRO_ISOT;
-  if(!strcmp(name, "disk")) return DISK;
-  return -1;
-}
+// A module's parameter names and reference defaults, in Params::par order
+// (the module's PARAMS: src/<module>/eqns.cuh).
+struct ParamTable {
+  const ParamDef *p;
+  int n;
+  const ParamDef *begin() const { return p; }
+  const ParamDef *end() const { return p + n; }
+  size_t size() const { return (size_t)n; }
+  const ParamDef &operator[](size_t k) const { return p[k]; }
+};
+ParamTable param_defs(int m) { return ParamTable{module_info(m).params, module_info(m).npar}; }
 
 // Largest modified wavenumber of -D11 (at kh = pi), and the RK3 stability
 // limits on the imaginary and negative real axes (|1+z+z^2/2+z^3/6| = 1).
@@ -261,15 +247,16 @@ SlabHalo slab_halo_of(const fg2_run *r)
 
 bool first_order_active(const fg2_run *r)
 {
-  if(r->module == HYDRO_ISOT) return (float)r->par[4] * (float)r->par[5] > 0.0f;  // f k > 0
-  if(r->module == DISK) return (float)r->par[10] != 0.0f;                          // ff != 0
-  return false;
+  float par[16] = {};
+  for(size_t k = 0; k < r->par.size() && k < 16; ++k) par[k] = (float)r->par[k];
+  return module_info(r->module).first_order_on(par);
 }
 
-// Draw the two random phases of one Euler() call (src/hydro-isot/Euler.cuh:39-44).
+// Draw the two random phases of one Euler() call of a module whose pre-step
+// takes them (src/hydro-isot/Euler.cuh:39-44).
 void randomize(fg2_run *r)
 {
-  if(r->module != HYDRO_ISOT) return;
+  if(!module_info(r->module).random_phases) return;
   const double u1 = (double)(splitmix64(r->rng) >> 11) * 0x1.0p-53;
   const double u2 = (double)(splitmix64(r->rng) >> 11) * 0x1.0p-53;
   r->rand1 = (float)(6.2831853071795865 * u1);
@@ -294,7 +281,7 @@ FusedShape shape_of(const fg2_run *r)
 // The one-pass step kernel serves the Cartesian modules, periodic or not.
 bool uses_step_kernel(const fg2_run *r)
 {
-  return r->path == FG2_PATH_FUSED && r->module != DISK;
+  return r->path == FG2_PATH_FUSED && module_info(r->module).one_pass;
 }
 
 FusedShape step_shape_of(const fg2_run *r)
@@ -338,24 +325,22 @@ FusedShape step_shape_of(const fg2_run *r)
   return sh;
 }
 
-// Diffusive rate of the Cartesian modules: D_max (k2/D1^2 + k2/D2^2).
+// Diffusive rate of a module with a constant D_max (its diffusivity()):
+// D_max (k2/D1^2 + k2/D2^2); a negative D_max means a per-cell rate.
 double lam_diff_const(const fg2_run *r)
 {
   const double k2s = KMAX2 * (1.0 / ((double)r->Delta1 * r->Delta1) +
                               1.0 / ((double)r->Delta2 * r->Delta2));
-  double dmax = 0.0;
-  if(r->module == HYDRO)
-    dmax = std::max({r->par[1] * 4.0 / 3.0 + r->par[2], (r->par[0] - 1.0) * r->par[3], r->par[4]});
-  else if(r->module == HYDRO_ISOT)
-    dmax = std::max(r->par[1] * 4.0 / 3.0 + r->par[2], r->par[0]);
-  return dmax * k2s;
+  return module_info(r->module).diffusivity(r->par.data()) * k2s;
 }
+
+bool cell_diffusivity(const fg2_run *r) { return module_info(r->module).diffusivity(r->par.data()) < 0.0; }
 
 double dt_from_rates(const fg2_run *r, double lam_adv, double lam_diff_cell)
 {
   const double k2s = KMAX2 * (1.0 / ((double)r->Delta1 * r->Delta1) +
                               1.0 / ((double)r->Delta2 * r->Delta2));
-  const double lam_diff = r->module == DISK ? lam_diff_cell * k2s : lam_diff_const(r);
+  const double lam_diff = cell_diffusivity(r) ? lam_diff_cell * k2s : lam_diff_const(r);
   double dt = INFINITY;
   if(lam_adv > 0) dt = std::min(dt, RK3_IMAG / lam_adv);
   if(lam_diff > 0) dt = std::min(dt, RK3_REAL / lam_diff);
@@ -371,7 +356,7 @@ ClockInit clock_of(const fg2_run *r, double t, double t_final, double fixed_dt)
   k.t_final = t_final;
   k.fixed_dt = fixed_dt;
   k.c = r->c;
-  k.lam_diff_const = r->module == DISK ? -1.0 : lam_diff_const(r);
+  k.lam_diff_const = cell_diffusivity(r) ? -1.0 : lam_diff_const(r);
   k.k2s = KMAX2 * (1.0 / ((double)r->Delta1 * r->Delta1) + 1.0 / ((double)r->Delta2 * r->Delta2));
   k.rng = r->rng;
   return k;
@@ -389,7 +374,7 @@ double bits_to_double(unsigned long long b)
 int refresh_rows(fg2_run *r)
 {
   if(!r->rowtab || !r->configured) return FG2_OK;
-  HIP_TRY(launch_disk_rows(r->rowtab, make_params(r), r->stream));
+  HIP_TRY(launch_rows(r->module, r->rowtab, make_params(r), r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
   return FG2_OK;
 }
@@ -435,6 +420,18 @@ extern "C" {
 
 const char *fg2_last_error(void) { return g_error.c_str(); }
 
+int fg2_module_count(void) { return module_count(); }
+
+int fg2_module(int k, const char **name, int *nvar, const char **fields)
+{
+  if(k < 0 || k >= module_count()) return fail(FG2_ERR_ARG, "no module %d", k);
+  const ModuleInfo &m = module_info(k);
+  if(name) *name = m.name;
+  if(nvar) *nvar = m.nv;
+  if(fields) *fields = m.fields;
+  return FG2_OK;
+}
+
 int fg2_setup(fg2_run **out, const char *module, int n1, int n2, size_t *bytes)
 {
   if(!out) return fail(FG2_ERR_ARG, "NULL output handle");
@@ -466,7 +463,8 @@ int fg2_setup(fg2_run **out, const char *module, int n1, int n2, size_t *bytes)
   if(e == hipSuccess) e = hipHostMalloc((void **)&r->host_diag, sizeof(DeviceDiag), hipHostMallocDefault);
   if(e == hipSuccess) e = hipMalloc((void **)&r->sum_partial, 4 * (size_t)n1 * sizeof(double));
   if(e == hipSuccess) e = hipMalloc((void **)&r->sum_out, 4 * sizeof(double));
-  if(e == hipSuccess && m == DISK) e = hipMalloc((void **)&r->rowtab, 4 * (size_t)n1 * sizeof(float));
+  if(e == hipSuccess && module_info(m).row_table)
+    e = hipMalloc((void **)&r->rowtab, 4 * (size_t)n1 * sizeof(float));
   int dev = 0;
   if(e == hipSuccess) e = hipGetDevice(&dev);
   if(e == hipSuccess) e = hipDeviceGetAttribute(&r->cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -524,15 +522,10 @@ int fg2_get_geometry(const fg2_run *r, double *out)
 int fg2_config(fg2_run *r)
 {
   if(!r) return fail(FG2_ERR_ARG, "NULL run");
-  // module settings(): src/hydro-isot/config.h:21-26, src/disk/config.h:21-34
-  if(r->module == HYDRO_ISOT) r->l1 = r->l2 = (float)(2.0 * M_PI);
-  if(r->module == DISK) {
-    r->l2 = (float)M_PI;
-    const float hd2 = (float)(0.5 * r->l2 / r->n2);
-    const float hd1 = (float)std::log(hd2 + std::sqrt(hd2 * (double)hd2 + 1.0));
-    r->l1 = (float)(2.0 * hd1 * r->n1);
-    r->p1 = 0; r->p2 = 0;
-  }
+  // the module's settings() (src/<module>/config.h)
+  Geometry g{r->l1, r->l2, r->p1, r->p2};
+  module_info(r->module).settings(g, r->n1, r->n2);
+  r->l1 = g.l1; r->l2 = g.l2; r->p1 = g.p1; r->p2 = g.p2;
   // Delta = l / n in R (src/config.cpp:39); a slab takes the whole grid's
   r->Delta1 = (float)r->l1 / (float)rows_of_grid(r);
   r->Delta2 = (float)r->l2 / (float)r->n2;
@@ -610,19 +603,7 @@ int fg2_download(fg2_run *r, float *h) { return r ? copy_interior(r, r->in[r->cu
 int fg2_init(fg2_run *r, const char *name, unsigned long long seed)
 {
   if(int e = check_run(r)) return e;
-  const std::string n = name ? name : "";
-  int ic = IC_ZEROS;                                 // the modules' pick() (init.h)
-  if(r->module == HYDRO) {
-    if(n == "blast") ic = IC_BLAST;
-    else if(n == "bow") ic = IC_BOW;
-    else if(n == "implos") ic = IC_IMPLOS;
-    else if(n == "KH") ic = IC_KH;
-    else if(n == "Sod") ic = IC_SOD;
-  } else if(r->module == HYDRO_ISOT) {
-    if(n == "Kf") ic = IC_KF;
-  } else {
-    ic = IC_HAWLEY;
-  }
+  const int ic = module_info(r->module).ic_of(name ? name : "");   // the module's pick() (init.h)
   const Params P = make_params(r);
   HIP_TRY(launch_ic(r->module, ic, r->in[r->cur], r->s, r->n1, r->n2, (float)r->l1, (float)r->l2,
                     P.par, seed, r->stream));
@@ -931,7 +912,9 @@ int fg2_slab(fg2_run *r, int row0, int n1_global)
     r->slab_row0 = r->slab_n1g = 0;
     return r->configured ? fg2_config(r) : FG2_OK;
   }
-  if(r->module == DISK) return fail(FG2_ERR_ARG, "disk grids are not periodic along dim 1: no slabs");
+  if(!module_info(r->module).one_pass || !r->p1)
+    return fail(FG2_ERR_ARG, "%s grid not periodic along dim 1 or stepped by substages: no slabs",
+                module_info(r->module).name);
   if(r->n1 < SLAB_HALO)
     return fail(FG2_ERR_SHAPE, "a slab needs at least %d rows, this run has %d", SLAB_HALO, r->n1);
   if(n1_global < r->n1 || row0 < 0 || row0 >= n1_global)
@@ -1097,10 +1080,10 @@ int fg2_vorticity(fg2_run *r, float *host)
 int fg2_perf_model(const fg2_run *r, double *flops, double *bits, double *alg)
 {
   if(!r) return fail(FG2_ERR_ARG, "NULL run");
-  static const int eqns_flop[3] = {289, 214, 733};   // src/*/config.h
   const double n = (double)r->n1 * r->n2;
   const double m1 = r->n1 + ORDER, m2 = r->n2 + ORDER;
-  if(flops) *flops = 3.0 * n * (eqns_flop[r->module] + r->nv * 2.0);           // src/config.cpp:33
+  // eqns_flop of the module (src/*/config.h), src/config.cpp:33
+  if(flops) *flops = 3.0 * n * (module_info(r->module).flop + r->nv * 2.0);
   if(bits) *bits = 3.0 * (m1 * m2 + n * 5.0 + (m1 + m2) * 2.0 * ORDER) * r->nv * sizeof(float) * 8;
   if(alg) *alg = 10.0 * r->nv * sizeof(float);
   return FG2_OK;
@@ -1325,7 +1308,7 @@ int fg2_batch_create(fg2_batch **out, fg2_run *const *runs, int nruns, int waves
     if(int e = check_run(runs[i])) { delete b; return e; }
     if(runs[i]->module != b->module) { delete b; return fail(FG2_ERR_ARG, "a batch holds runs of one module"); }
     if(runs[i]->slab_n1g) { delete b; return fail(FG2_ERR_ARG, "a batch holds whole grids, not slabs"); }
-    b->step = b->step && runs[i]->p1 && runs[i]->p2 && runs[i]->module != DISK;
+    b->step = b->step && runs[i]->p1 && runs[i]->p2 && module_info(runs[i]->module).one_pass;
   }
   for(int i = 0; i < nruns; ++i) {
     b->runs.push_back(runs[i]);
@@ -1490,7 +1473,8 @@ int fg2_batch_graph_run(fg2_batch *b, const double *dts, int steps, int replays)
   // runs with a hydro-isot forcing draw new phases every step: the device
   // clock draws them (from each run's own stream) with the fixed dts
   bool forced = false;
-  for(fg2_run *r : b->runs) forced = forced || (r->module == HYDRO_ISOT && first_order_active(r));
+  for(fg2_run *r : b->runs)
+    forced = forced || (module_info(r->module).random_phases && first_order_active(r));
   b->clocked = forced;
   if(forced) {
     // the clock draws the phases of every step of the call; the host's
@@ -1642,7 +1626,7 @@ extern "C" int fg2_batch_step_static(fg2_run *const *runs, int nruns, const doub
     if(int e = check_run(r)) return e;
     if(r->module != module) return fail(FG2_ERR_ARG, "a batch holds runs of one module");
     if(r->slab_n1g) return fail(FG2_ERR_ARG, "a batch holds whole grids, not slabs");
-    step = step && r->p1 && r->p2 && r->module != DISK;
+    step = step && r->p1 && r->p2 && module_info(r->module).one_pass;
   }
   std::lock_guard<std::mutex> lk(g_static_mu);
   unsigned char *base = nullptr;
@@ -1929,7 +1913,7 @@ int server_launch_per_run(fg2_server *S)
 int server_launch(fg2_server *S)
 {
   if(S->mode == FG2_SERVE_PER_RUN) return server_launch_per_run(S);
-  for(int m = 0; m < 2; ++m) {
+  for(int m = 0; m < module_count(); ++m) {        // one launch per module with active runs
     S->rec_host.clear();
     const FusedShape sh = server_shape(S, m);
     const int th = sh.tile_h;
@@ -1950,7 +1934,7 @@ int server_launch(fg2_server *S)
       R.diag = S->diags + q->diag_slot;
       S->rec_host.push_back(R);
       // CFL steps, and forcing phases drawn every step, come from the clock
-      clocked = clocked || q->cfl || (q->fo && m == HYDRO_ISOT);
+      clocked = clocked || q->cfl || (q->fo && module_info(m).random_phases);
       cfl = cfl || q->cfl;
     }
     if(S->rec_host.empty()) continue;
@@ -2115,7 +2099,8 @@ int fg2_server_submit(fg2_server *S, const char *module, int n1, int n2, int ste
 {
   if(!S || !init || steps < 1) return fail(FG2_ERR_ARG, "bad request");
   const int m = module_id(module);
-  if(m != HYDRO && m != HYDRO_ISOT) return fail(FG2_ERR_ARG, "served modules: hydro, hydro-isot");
+  if(m < 0 || !module_info(m).one_pass)
+    return fail(FG2_ERR_ARG, "served runs take the one-pass step (module %s)", module ? module : "(null)");
   if(n1 < HALF || n2 < HALF) return fail(FG2_ERR_SHAPE, "grid %d x %d too small", n1, n2);
   // geometry, parameters and dt of a default run of this module (periodic,
   // src/global.cpp:28-29 and the module's settings()), no device memory
@@ -2132,7 +2117,8 @@ int fg2_server_submit_run(fg2_server *S, const fg2_run *like, int steps, const f
   if(!S || !like || !init || steps < 1) return fail(FG2_ERR_ARG, "bad request");
   if(int e = check_run(like)) return e;
   const int m = like->module;
-  if(m != HYDRO && m != HYDRO_ISOT) return fail(FG2_ERR_ARG, "served modules: hydro, hydro-isot");
+  if(!module_info(m).one_pass)
+    return fail(FG2_ERR_ARG, "served runs take the one-pass step (module %s)", module_info(m).name);
   if(!like->p1 || !like->p2) return fail(FG2_ERR_ARG, "served runs are periodic");
   if(like->slab_n1g) return fail(FG2_ERR_ARG, "served runs are whole grids, not slabs");
   const int n1 = like->n1, n2 = like->n2;

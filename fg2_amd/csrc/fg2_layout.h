@@ -19,10 +19,6 @@ constexpr int ORDER   = 6;
 constexpr int ROW_PAD = 64;  // floats before interior column 0 in each row (256 B)
 constexpr int STRIP   = 64 - 2 * HALF;  // output columns of one wavefront tile
 
-enum ModuleId { HYDRO = 0, HYDRO_ISOT = 1, DISK = 2 };
-
-inline constexpr int nvar_of(int m) { return m == HYDRO ? 4 : m == HYDRO_ISOT ? 3 : 6; }
-
 // Row stride in floats: ROW_PAD + (n2 + HALF) * nvar, rounded up to 64
 // floats so that every row starts on a 256-byte boundary.
 inline constexpr long row_stride(int n2, int nv)
@@ -60,9 +56,9 @@ struct Params {
   float par[16];
   // hydro-isot first_order random phases (src/hydro-isot/Euler.cuh:24-25)
   float rand1, rand2;
-  // disk: per-row table of the run, 4 floats per row i:
-  // {sphr = rin e^{(i+1/2) Delta1}, 1/sphr, sphr^2, gravity M / (sphr - rS)^2
-  //  (softened when the parameter eps > 0)}
+  // per-row table of a module that reads one (ROW_TABLE), 4 floats per row
+  // from the module's row_entry(); disk: {sphr = rin e^{(i+1/2) Delta1},
+  // 1/sphr, sphr^2, gravity}
   const float *rowtab;
 };
 

@@ -843,23 +843,7 @@ void sums_row_kernel(const float *u, const long s, const Params P, double *parti
   const int i = blockIdx.x;
   double acc[4] = {0, 0, 0, 0};
   for(int j = threadIdx.x; j < P.n2; j += blockDim.x) {
-    const float *c = u + (long)i * s + (long)j * M::NV;
-    const double rho = exp((double)c[0]);
-    if constexpr(M::ID == DISK) {
-      const double sphr = (double)P.par[2] * exp((i + 0.5) * (double)P.Delta1);
-      const double th = (j + 0.5) * (double)P.Delta2;
-      const double R = sphr * sin(th);
-      // volume element of the (log r, theta) grid: r^3 sin(theta) dlnr dtheta
-      const double dV = sphr * sphr * sphr * sin(th);
-      acc[0] += rho * dV;
-      acc[1] += rho * c[1] * dV;
-      acc[2] += rho * c[2] * dV;
-      acc[3] += rho * c[3] * R * R * dV;
-    } else {
-      acc[0] += rho;
-      acc[1] += rho * c[1];
-      acc[2] += rho * c[2];
-    }
+    M::sums(P, u + (long)i * s + (long)j * M::NV, i, j, acc);
   }
   __shared__ double red[4][256];
 #pragma unroll
@@ -895,15 +879,6 @@ void sums_final_kernel(const double *partial, const int n1, double *out)
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template<class F> static hipError_t by_module(int module, F f)
-{
-  switch(module) {
-  case HYDRO:      return f(Hydro{});
-  case HYDRO_ISOT: return f(HydroIsot{});
-  case DISK:       return f(Disk{});
-  }
-  return hipErrorInvalidValue;
-}
 
 hipError_t launch_bcond(int module, float *u, long s, const Params &P, int p1, int p2, hipStream_t st)
 {
@@ -938,34 +913,28 @@ hipError_t launch_drift(int module, float *u, const float *v, long s, const Para
   });
 }
 
-// Per-row table of the disk grid (Params::rowtab): the row's spherical
-// radius (src/disk/eqns.cuh:52-54) and the gravity of the point mass
-// (src/disk/eqns.cuh:238-239), in the reference's expressions.  A softening
-// length eps > 0 (an added parameter; the reference has none) replaces
-// M / d^2 by the Plummer force M d / (d^2 + eps^2)^(3/2), d = sphr - rS.
+// Per-row table (Params::rowtab) of a module that reads one: the module's
+// row_entry() for every row.
+template<class M>
 __global__ __launch_bounds__(256)
-void disk_rows_kernel(float *tab, const int n1, const float Delta1, const float M, const float rS,
-                      const float rin, const float eps)
+void rows_kernel(float *tab, const Params P)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if(i >= n1) return;
-  const float sphr = rin * expf(((float)i + 0.5f) * Delta1);
-  const float tmp = sphr - rS;
-  float gr;
-  if(eps == 0.0f) {
-    gr = M / (tmp * tmp);
-  } else {
-    const float d2 = tmp * tmp + eps * eps;
-    gr = M * tmp / (d2 * sqrtf(d2));
-  }
-  reinterpret_cast<float4 *>(tab)[i] = make_float4(sphr, 1.0f / sphr, sphr * sphr, gr);
+  if(i >= P.n1) return;
+  reinterpret_cast<float4 *>(tab)[i] = M::row_entry(P, i);
 }
 
-hipError_t launch_disk_rows(float *tab, const Params &P, hipStream_t st)
+hipError_t launch_rows(int module, float *tab, const Params &P, hipStream_t st)
 {
-  disk_rows_kernel<<<(P.n1 + 255) / 256, 256, 0, st>>>(tab, P.n1, P.Delta1, P.par[0], P.par[1],
-                                                       P.par[2], P.par[15]);
-  return hipGetLastError();
+  return by_module(module, [&](auto m) {
+    using M = decltype(m);
+    if constexpr(!M::ROW_TABLE) {
+      return hipSuccess;
+    } else {
+      rows_kernel<M><<<(P.n1 + 255) / 256, 256, 0, st>>>(tab, P);
+      return hipGetLastError();
+    }
+  });
 }
 
 hipError_t launch_euler(int module, float *u, long s, const Params &P, float dt, hipStream_t st)
@@ -1038,7 +1007,7 @@ hipError_t launch_step(int module, const float *uin, float *uout, long s, const 
   const float b1 = (float)(-5.0 / 9.0), b2 = (float)(-153.0 / 128.0);
   return by_module(module, [&](auto m) {
     using M = decltype(m);
-    if constexpr(M::ID == DISK) {
+    if constexpr(!M::ONE_PASS) {
       return hipErrorInvalidValue;   // disk runs take the substage path
     } else {
     const int tile_h = std::max(1, sh.tile_h);
@@ -1103,7 +1072,7 @@ hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
       FG2_W(1) FG2_W(2) FG2_W(4)
 #undef FG2_W
     } else {
-      if constexpr(M::ID == DISK) {
+      if constexpr(!M::ONE_PASS) {
         return hipErrorInvalidValue;
       } else {
         if(kind == 1) {
@@ -1135,7 +1104,7 @@ hipError_t launch_batch_step(int module, const BatchRun *runs_dev, int nruns, in
 {
   return by_module(module, [&](auto m) {
     using M = decltype(m);
-    if constexpr(M::ID == DISK) {
+    if constexpr(!M::ONE_PASS) {
       return hipErrorInvalidValue;
     } else {
       switch(waves) {
@@ -1311,7 +1280,7 @@ hipError_t launch_clock(int module, BatchRun *set0, BatchRun *set1, BatchRun *se
 {
   if(nruns < 1) return hipSuccess;
   clock_kernel<<<(nruns + 255) / 256, 256, 0, st>>>(set0, set1, set2, nsets, step, nruns,
-                                                     module == HYDRO_ISOT);
+                                                     module_info(module).random_phases);
   return hipGetLastError();
 }
 

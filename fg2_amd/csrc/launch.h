@@ -57,9 +57,9 @@ inline long step_launch_tiles(int n1, int n2, int tile_h, int waves, int *nfull)
   return (long)full * nchunks + (pack ? (nchunks + waves - 1) / waves : 0);
 }
 
-// Fill the disk row table (Params::rowtab, 4 floats per row) of a run
-// from its Delta1 and parameters M, rS, rin.
-hipError_t launch_disk_rows(float *tab, const Params &P, hipStream_t st);
+// Fill the per-row table (Params::rowtab, 4 floats per row) of a run of a
+// module that reads one (ROW_TABLE: disk, from Delta1 and M, rS, rin, eps).
+hipError_t launch_rows(int module, float *tab, const Params &P, hipStream_t st);
 
 // Resident workgroups per CU (hipOccupancyMaxActiveBlocksPerMultiprocessor)
 // with `waves` wavefronts per workgroup of the substage kernels (kind 0, the
@@ -81,8 +81,8 @@ hipError_t launch_sums(int module, const float *u, long s, const Params &P, doub
                        double *out, hipStream_t st);
 
 // Initial conditions on the device (ics.hip); par = the run's 16 FP32
-// module parameters.  ic selects the function of the module's init.h.
-enum IcId { IC_ZEROS = 0, IC_BLAST, IC_BOW, IC_IMPLOS, IC_KH, IC_SOD, IC_KF, IC_HAWLEY };
+// module parameters.  ic: the module's own number of a named condition
+// (its ic_of(); 0 = zeros).
 hipError_t launch_ic(int module, int ic, float *u, long s, int n1, int n2, float l1, float l2,
                      const float *par, unsigned long long seed, hipStream_t st);
 

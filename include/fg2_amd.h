@@ -55,6 +55,14 @@ enum {
 /* Text of the last error on this host thread (never NULL). */
 const char *fg2_last_error(void);
 
+/* The physics modules the library is built with (the reference builds one,
+ * chosen by its Makefile's -Isrc/<module>, Makefile:19-22): the count, and
+ * for module k its name (the argument of fg2_setup), NVAR and its state
+ * variables, comma-separated in storage order (src/<module>/state.h).
+ * fg2_module returns FG2_ERR_ARG for k out of range; outputs may be NULL. */
+int fg2_module_count(void);
+int fg2_module(int k, const char **name, int *nvar, const char **fields);
+
 /* ---- set-up: src/setup.cpp:32-87, src/config.cpp:24-41, src/para.cpp:25-40 ---- */
 
 /* Allocate a run of `module` ("hydro", "hydro-isot" or "disk") on n1 x n2

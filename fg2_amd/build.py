@@ -17,6 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfg2_amd.so")
 COPY_EXE = os.path.join(HERE, "hbm_copy")
+VALU_EXE = os.path.join(HERE, "valu_peak")
 DRIVER_EXE = os.path.join(HERE, "fg2run")
 SOURCES = ["kernels.hip", "capi.hip", "ics.hip", "spectra.hip"]
 HEADERS = ["fg2_layout.h", "stencil.h", "physics.h", "launch.h", "modules/hydro.h",
@@ -57,6 +58,11 @@ def build(verbose=False, jobs=2):
     os.replace(tmp, LIB)
     # the HBM copy calibration executable
     cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "hbm_copy.hip"), "-o", COPY_EXE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    # the VALU issue ceiling probe
+    cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "valu_peak.hip"), "-o", VALU_EXE]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)

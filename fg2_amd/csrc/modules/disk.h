@@ -93,23 +93,28 @@ struct Disk {
     const float temp = gamma1 * eng;
     const float nuSS = alpha * g * temp * r * __builtin_amdgcn_sqrtf(r * __builtin_amdgcn_rcpf(M));
 
+    // d/dr = (sin d/dlnr + cos d/dtheta) / sphr, d/dz = (cos d/dlnr - sin
+    // d/dtheta) / sphr, with 1/sphr folded into the weights
+    const float cr = cost * isphr, sr = sint * isphr;
     float dr[NV], dz[NV], lap[NV], d11[NV], d22[NV];
 #pragma unroll
     for(int q = 0; q < NV; ++q) {
       const float a = D1(P, x, q), b = D2(P, x, q);
-      dr[q] = (sint * a + cost * b) * isphr;
-      dz[q] = (cost * a - sint * b) * isphr;
-      d11[q] = D11(P, x, q);
-      d22[q] = D22(P, x, q);
+      dr[q] = sr * a + cr * b;
+      dz[q] = cr * a - sr * b;
+      if(q != OMG && q != LNE) {          // Omg and lne need only their Laplacian
+        d11[q] = D11(P, x, q);
+        d22[q] = D22(P, x, q);
+      }
     }
     const float d12_lnd = D12(P, x, 0), d12_ur = D12(P, x, 1);
     const float d12_uz = D12(P, x, 2), d12_lner = D12(P, x, 3);
-    const float cr = cost * isphr, sr = sint * isphr;
     const float cc = cr * cr, cs = cr * sr, ss = sr * sr;
     const float c2 = cc - ss, s2 = 2.0f * cs;
     const float isphr_2 = isphr * isphr;
 #pragma unroll
-    for(int q = 0; q < NV; ++q) lap[q] = (d11[q] + d22[q]) * isphr_2;
+    for(int q = 0; q < NV; ++q)
+      lap[q] = (q == OMG || q == LNE ? LAP(P, x, q) : d11[q] + d22[q]) * isphr_2;
 
     const float drr_lnd  = ss * d11[LND]  + cc * d22[LND]  + s2 * d12_lnd  + cr * dz[LND]  - sr * dr[LND];
     const float drr_ur   = ss * d11[UR]   + cc * d22[UR]   + s2 * d12_ur   + cr * dz[UR]   - sr * dr[UR];

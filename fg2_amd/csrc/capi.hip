@@ -1723,6 +1723,11 @@ struct fg2_server {
   size_t budget = 0, in_use = 0;
   int waves = 0, tile_h = 0, cus = 256;        // 0: chosen per launch (server_shape)
   hipStream_t stream = nullptr;
+  // one compute stream per module (mstream[0] is `stream`): the batched
+  // launches of the modules of a tick run side by side, each filling the
+  // slots the other's last round leaves idle
+  static constexpr int MAXM = 8;
+  hipStream_t mstream[MAXM] = {};
   char *arena = nullptr;
   std::map<size_t, size_t> free_list;          // offset -> size
   DeviceDiag *diags = nullptr;
@@ -1758,6 +1763,8 @@ struct fg2_server {
   hipEvent_t tick_ev[SLOTS][NPOOL] = {};
   int tick_slot = 0, next_sidx = 0;
 };
+
+static_assert(Modules::size <= fg2_server::MAXM, "one compute stream per module");
 
 namespace {
 
@@ -1808,7 +1815,7 @@ int server_grow(fg2_server *S, size_t need)
 {
   if(need <= S->rec_cap) return FG2_OK;
   size_t cap = std::max<size_t>(need, 2 * S->rec_cap + 64);
-  HIP_TRY(hipStreamSynchronize(S->stream));
+  for(int m = 0; m < module_count(); ++m) HIP_TRY(hipStreamSynchronize(S->mstream[m]));
   for(int k = 0; k < fg2_server::SLOTS; ++k) {
     if(S->rec_dev[k]) HIP_TRY(hipFree(S->rec_dev[k]));
     if(S->rec_pinned[k]) HIP_TRY(hipHostFree(S->rec_pinned[k]));
@@ -1822,7 +1829,7 @@ int server_grow(fg2_server *S, size_t need)
 // The stream a request's steps run on.
 hipStream_t run_stream(const fg2_server *S, const Request *q)
 {
-  return S->mode == FG2_SERVE_PER_RUN ? S->pool[q->sidx] : S->stream;
+  return S->mode == FG2_SERVE_PER_RUN ? S->pool[q->sidx] : S->mstream[q->module];
 }
 
 int server_admit(fg2_server *S, double now)
@@ -1914,6 +1921,7 @@ int server_launch(fg2_server *S)
 {
   if(S->mode == FG2_SERVE_PER_RUN) return server_launch_per_run(S);
   for(int m = 0; m < module_count(); ++m) {        // one launch per module with active runs
+    hipStream_t ms = S->mstream[m];
     S->rec_host.clear();
     const FusedShape sh = server_shape(S, m);
     const int th = sh.tile_h;
@@ -1946,16 +1954,16 @@ int server_launch(fg2_server *S)
     HIP_TRY(hipEventSynchronize(S->rec_done[slot]));   // the slot's previous launch is done
     memcpy(S->rec_pinned[slot], S->rec_host.data(), S->rec_host.size() * sizeof(BatchRun));
     HIP_TRY(hipMemcpyAsync(S->rec_dev[slot], S->rec_pinned[slot], S->rec_host.size() * sizeof(BatchRun),
-                           hipMemcpyHostToDevice, S->stream));
+                           hipMemcpyHostToDevice, ms));
     const int n = (int)S->rec_host.size();
     if(clocked)
-      HIP_TRY(launch_clock(m, S->rec_dev[slot], nullptr, nullptr, 1, 1, n, S->stream));
-    if(fo_blocks) HIP_TRY(launch_batch_euler(m, S->rec_dev[slot], n, fo_blocks, S->stream));
-    zero_ints_kernel<<<1, 64, 0, S->stream>>>(S->counters + slot, 1);
+      HIP_TRY(launch_clock(m, S->rec_dev[slot], nullptr, nullptr, 1, 1, n, ms));
+    if(fo_blocks) HIP_TRY(launch_batch_euler(m, S->rec_dev[slot], n, fo_blocks, ms));
+    zero_ints_kernel<<<1, 64, 0, ms>>>(S->counters + slot, 1);
     HIP_TRY(launch_batch_step(m, S->rec_dev[slot], n, begin, sh.waves,
                               batch_grid(m, true, sh.waves, S->cus, cfl), S->counters + slot,
-                              cfl, S->stream));
-    HIP_TRY(hipEventRecord(S->rec_done[slot], S->stream));
+                              cfl, ms));
+    HIP_TRY(hipEventRecord(S->rec_done[slot], ms));
   }
   for(Request *q : S->active) {
     if(q->done_steps >= q->steps) continue;
@@ -2030,6 +2038,9 @@ int fg2_server_create(fg2_server **out, size_t hbm_budget, int waves, int tile_h
   S->ndiag = 8192;
   int dev = 0;
   hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  S->mstream[0] = S->stream;
+  for(int m = 1; m < module_count() && m < fg2_server::MAXM && e == hipSuccess; ++m)
+    e = hipStreamCreateWithFlags(&S->mstream[m], hipStreamNonBlocking);
   if(e == hipSuccess) e = hipGetDevice(&dev);
   if(e == hipSuccess) e = hipDeviceGetAttribute(&S->cus, hipDeviceAttributeMultiprocessorCount, dev);
   if(e == hipSuccess) e = hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking);
@@ -2066,7 +2077,7 @@ int fg2_server_create(fg2_server **out, size_t hbm_budget, int waves, int tile_h
 void fg2_server_free(fg2_server *S)
 {
   if(!S) return;
-  if(S->stream) (void)hipStreamSynchronize(S->stream);
+  for(int m = 0; m < fg2_server::MAXM; ++m) if(S->mstream[m]) (void)hipStreamSynchronize(S->mstream[m]);
   if(S->copy) (void)hipStreamSynchronize(S->copy);
   for(int j = 0; j < fg2_server::NPOOL; ++j) {
     if(S->pool[j]) (void)hipStreamSynchronize(S->pool[j]);
@@ -2090,6 +2101,7 @@ void fg2_server_free(fg2_server *S)
   if(S->diags) (void)hipFree(S->diags);
   if(S->diags_host) (void)hipHostFree(S->diags_host);
   if(S->counters) (void)hipFree(S->counters);
+  for(int m = 1; m < fg2_server::MAXM; ++m) if(S->mstream[m]) (void)hipStreamDestroy(S->mstream[m]);
   if(S->stream) (void)hipStreamDestroy(S->stream);
   delete S;
 }
@@ -2157,7 +2169,7 @@ int fg2_server_tick(fg2_server *S, double now, int *active, int *queued, int *fi
   if(int e = server_launch(S)) return e;
   if(int e = server_retire(S, now)) return e;
   if(S->active.empty() && S->queue.empty()) {
-    HIP_TRY(hipStreamSynchronize(S->stream));
+    for(int m = 0; m < module_count(); ++m) HIP_TRY(hipStreamSynchronize(S->mstream[m]));
     HIP_TRY(hipStreamSynchronize(S->copy));
     for(int j = 0; j < fg2_server::NPOOL; ++j) HIP_TRY(hipStreamSynchronize(S->pool[j]));
   }

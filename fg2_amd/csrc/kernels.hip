@@ -282,8 +282,6 @@ void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, fl
 // index-mapped read, bit for bit.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int pmod(int a, int n) { const int r = a % n; return r < 0 ? r + n : r; }
-// pmod for a in [-n, 2n): one compare and add instead of a division
-__device__ __forceinline__ int wrap_once(int a, int n) { return a < 0 ? a + n : a >= n ? a - n : a; }
 // Mirror of an index about the faces of [0, n), clamped into [0, n) for
 // indices more than n outside (those feed only discarded values).
 __device__ __forceinline__ int mirror(int a, int n)
@@ -395,9 +393,6 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
   }
 
   float W0[8][NV], W1[8][NV], W2[8][NV], V0[4][NV], V1[4][NV];
-  // the rows this tile loads are [i0 - 9, i0 + nit - 3] with i0 < n1: inside
-  // [-n1, 2 n1) when n1 >= 9 and nit - 3 <= n1, so one wrap maps them
-  const bool wrap1 = n1 >= 3 * HALF && nit - 3 <= n1;
   auto load_row = [&](int r, float (&dst)[NV]) {
     if constexpr(SLAB) {
       // r >= i0 - 9 >= -9 always; the last prefetch asks for row i1 + 9,
@@ -408,7 +403,7 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
     } else if constexpr(BOUNDED) {
       load_cell<NV>(col + (long)(mirror_rows ? mirror(r, n1) : pmod(r, n1)) * s, dst);
     } else {
-      load_cell<NV>(col + (long)(wrap1 ? wrap_once(r, n1) : pmod(r, n1)) * s, dst);
+      load_cell<NV>(col + (long)pmod(r, n1) * s, dst);
     }
   };
 #pragma unroll

@@ -1690,6 +1690,10 @@ extern "C" int fg2_batch_step_static(fg2_run *const *runs, int nruns, const doub
 // or hipFree calls; a served run holds only its two state arrays (the one-
 // pass step needs no v).  Launch records are quadruple-buffered in pinned
 // memory behind events, so the host prepares the next ticks while one runs.
+// States move between the caller's host arrays and the arena through pinned
+// staging buffers; the host-side copies into and out of them are cut into
+// row blocks of at most a per-tick byte budget, so admitting or retiring a
+// large run never holds up the ticks that keep the device busy.
 // Each launch's tile height and workgroup shape come from the active runs
 // (server_shape) unless fixed at creation.
 // A run's arithmetic is that of fg2_step (the same step tile), so a result
@@ -1714,7 +1718,9 @@ struct Request {
   int sidx = 0;                        // per-run mode: the pool stream it steps on
   int status = FG2_OK;
   bool active = false, retiring = false;
-  hipEvent_t copied = nullptr;
+  int up_slot = -1, up_rows = 0;       // staging slot and rows sent while uploading
+  int down_slot = -1, down_rows = 0;   // staging slot and rows copied out while retiring
+  bool down_landed = false;            // the state and flags are in the staging slot
 };
 
 } // namespace
@@ -1735,7 +1741,7 @@ struct fg2_server {
   int ndiag = 0;
   std::vector<int> free_diag;
   long long next_id = 0;
-  std::vector<Request *> queue, active, finished;
+  std::vector<Request *> queue, uploading, active, finished;
   std::map<long long, Request *> all;
   static constexpr int SLOTS = 4;              // launch records in flight
   BatchRun *rec_dev[SLOTS] = {};
@@ -1753,8 +1759,9 @@ struct fg2_server {
   float *up[STAGES] = {}, *down[STAGES] = {};
   size_t up_cap[STAGES] = {}, down_cap[STAGES] = {};
   hipEvent_t up_done[STAGES] = {}, down_done[STAGES] = {};
-  Request *down_owner[STAGES] = {};
-  int up_slot = 0;
+  Request *up_owner[STAGES] = {}, *down_owner[STAGES] = {};
+  // host bytes copied into or out of the staging buffers per tick
+  static constexpr size_t COPY_BUDGET = (size_t)8 << 20;
   // per-run mode (FG2_SERVE_PER_RUN): each active run launches its own
   // one-pass step on one of NPOOL streams; at most SLOTS ticks in flight
   int mode = FG2_SERVE_BATCHED;
@@ -1832,13 +1839,49 @@ hipStream_t run_stream(const fg2_server *S, const Request *q)
   return S->mode == FG2_SERVE_PER_RUN ? S->pool[q->sidx] : S->mstream[q->module];
 }
 
-int server_admit(fg2_server *S, double now)
+// Bytes of one row of a request's state in its host array.
+size_t host_row_bytes(const Request *q) { return (size_t)q->n2 * q->nv * sizeof(float); }
+
+// Copy rows [*rows, n1) of a state between a host array and a pinned
+// staging buffer (to_stage: host -> staging) within *budget bytes, at least
+// one row when any budget is left; returns the first row copied.  For an
+// upload, each block is then queued to the device by the caller.
+int stage_rows(const Request *q, char *stage, char *host, bool to_stage, int *rows,
+               size_t *budget)
+{
+  const size_t hp = host_row_bytes(q);
+  const int r0 = *rows;
+  if(r0 >= q->n1 || *budget == 0) return r0;
+  const int r1 = (int)std::min<size_t>((size_t)q->n1, (size_t)r0 + std::max<size_t>(1, *budget / hp));
+  const size_t bytes = (size_t)(r1 - r0) * hp, at = (size_t)r0 * hp;
+  if(to_stage) memcpy(stage + at, host + at, bytes);
+  else memcpy(host + at, stage + at, bytes);
+  *budget -= std::min(*budget, bytes);
+  *rows = r1;
+  return r0;
+}
+
+// Admission.  Queued requests, in arrival order, take arena space, a flag
+// slot and a staging buffer whose previous upload has reached the device.
+// Uploading runs then send rows within *budget host bytes: each block is
+// copied into the staging buffer and queued to the device on the copy
+// stream.  A run whose rows are all sent becomes active; its steps wait for
+// the upload on its compute stream.
+int server_admit(fg2_server *S, double now, size_t *budget)
 {
   while(!S->queue.empty() && !S->free_diag.empty()) {
+    int u = -1;
+    for(int k = 0; k < fg2_server::STAGES && u < 0; ++k)
+      if(!S->up_owner[k] && hipEventQuery(S->up_done[k]) == hipSuccess) u = k;
+    if(u < 0) break;                                   // every staging buffer is in use
     Request *q = S->queue.front();
     const size_t one = (size_t)array_floats(q->n1, q->n2, q->nv) * sizeof(float);
     size_t off;
     if(!arena_alloc(S, 2 * one, &off)) break;          // wait for retirements
+    if(int e = stage_buffer(&S->up[u], &S->up_cap[u], host_row_bytes(q) * q->n1)) {
+      arena_free(S, off, 2 * one);
+      return e;
+    }
     q->off = off;
     q->bytes = 2 * one;
     q->s = row_stride(q->n2, q->nv);
@@ -1848,15 +1891,28 @@ int server_admit(fg2_server *S, double now)
     q->diag_slot = S->free_diag.back();
     S->free_diag.pop_back();
     set_clock_kernel<<<1, 1, 0, S->copy>>>(S->diags + q->diag_slot, q->clk, 1);
-    const size_t hp = (size_t)q->n2 * q->nv * sizeof(float);
-    const int u = S->up_slot;
-    S->up_slot = (S->up_slot + 1) % fg2_server::STAGES;
-    HIP_TRY(hipEventSynchronize(S->up_done[u]));                // staging slot free
-    if(int e = stage_buffer(&S->up[u], &S->up_cap[u], hp * q->n1)) return e;
-    memcpy(S->up[u], q->host.data(), hp * q->n1);
-    HIP_TRY(hipMemcpy2DAsync(q->in[0], q->s * sizeof(float), S->up[u], hp, hp, q->n1,
-                             hipMemcpyHostToDevice, S->copy));
+    q->up_slot = u;
+    q->up_rows = 0;
+    S->up_owner[u] = q;
+    S->in_use += q->bytes;
+    S->queue.erase(S->queue.begin());
+    S->uploading.push_back(q);
+  }
+  while(!S->uploading.empty()) {
+    Request *q = S->uploading.front();
+    const int u = q->up_slot;
+    const size_t hp = host_row_bytes(q);
+    char *stage = (char *)S->up[u];
+    while(q->up_rows < q->n1 && *budget > 0) {
+      const int r0 = stage_rows(q, stage, (char *)q->host.data(), true, &q->up_rows, budget);
+      HIP_TRY(hipMemcpy2DAsync(q->in[0] + (long)r0 * q->s, q->s * sizeof(float),
+                               stage + (size_t)r0 * hp, hp, hp, q->up_rows - r0,
+                               hipMemcpyHostToDevice, S->copy));
+    }
+    if(q->up_rows < q->n1) break;                      // the budget is spent
     HIP_TRY(hipEventRecord(S->up_done[u], S->copy));
+    S->up_owner[u] = nullptr;
+    q->up_slot = -1;
     if(S->mode == FG2_SERVE_PER_RUN) {
       q->sidx = S->next_sidx;
       S->next_sidx = (S->next_sidx + 1) % fg2_server::NPOOL;
@@ -1866,8 +1922,7 @@ int server_admit(fg2_server *S, double now)
       HIP_TRY(launch_cfl(q->module, q->in[0], q->s, q->P, S->diags + q->diag_slot, run_stream(S, q)));
     q->t_admit = now;
     q->active = true;
-    S->in_use += q->bytes;
-    S->queue.erase(S->queue.begin());
+    S->uploading.erase(S->uploading.begin());
     S->active.push_back(q);
   }
   return FG2_OK;
@@ -1974,16 +2029,16 @@ int server_launch(fg2_server *S)
   return FG2_OK;
 }
 
-int server_retire(fg2_server *S, double now)
+int server_retire(fg2_server *S, double now, size_t *budget)
 {
   // queue the copy-back (on the copy stream, after the run's last step) of
-  // runs that took their last step, one pinned staging slot each
+  // runs that took their last step, one pinned staging buffer each
   for(Request *q : S->active)
     if(!q->retiring && q->done_steps >= q->steps) {
       int d = -1;
-      for(int k = 0; k < fg2_server::STAGES; ++k) if(!S->down_owner[k]) { d = k; break; }
+      for(int k = 0; k < fg2_server::STAGES && d < 0; ++k) if(!S->down_owner[k]) d = k;
       if(d < 0) break;                                          // retry next tick
-      const size_t hp = (size_t)q->n2 * q->nv * sizeof(float);
+      const size_t hp = host_row_bytes(q);
       if(int e = stage_buffer(&S->down[d], &S->down_cap[d], hp * q->n1)) return e;
       HIP_TRY(hipEventRecord(S->down_done[d], run_stream(S, q)));  // the run's last step
       HIP_TRY(hipStreamWaitEvent(S->copy, S->down_done[d], 0));
@@ -1993,30 +2048,36 @@ int server_retire(fg2_server *S, double now)
                              sizeof(DeviceDiag), hipMemcpyDeviceToHost, S->copy));
       HIP_TRY(hipEventRecord(S->down_done[d], S->copy));
       S->down_owner[d] = q;
-      q->copied = S->down_done[d];
+      q->down_slot = d;
+      q->down_rows = 0;
+      q->down_landed = false;
       q->retiring = true;
     }
-  // release the ones whose copy has completed
+  // a run whose copy-back has landed releases its arena space and flag slot
+  // at once, then copies its rows out of the staging buffer within *budget
+  // host bytes per tick; it is finished when the last row is out
   for(size_t k = 0; k < S->active.size();) {
     Request *q = S->active[k];
-    if(q->retiring && hipEventQuery(q->copied) == hipSuccess) {
-      int d = 0;
-      while(S->down_owner[d] != q) ++d;
-      memcpy(q->host.data(), S->down[d], q->host.size() * sizeof(float));
-      S->down_owner[d] = nullptr;
+    if(!q->retiring) { ++k; continue; }
+    const int d = q->down_slot;
+    if(!q->down_landed) {
+      if(hipEventQuery(S->down_done[d]) != hipSuccess) { ++k; continue; }
+      q->down_landed = true;
       const unsigned int fl = S->diags_host[q->diag_slot].flags;
       q->status = (fl & 1u) ? FG2_ERR_NAN : (fl & 2u) ? FG2_ERR_ARG : FG2_OK;
-      q->t_done = now;
       arena_free(S, q->off, q->bytes);
       S->in_use -= q->bytes;
       S->free_diag.push_back(q->diag_slot);
-      q->active = false;
-      q->copied = nullptr;
-      S->active.erase(S->active.begin() + k);
-      S->finished.push_back(q);
-    } else {
-      ++k;
     }
+    while(q->down_rows < q->n1 && *budget > 0)
+      stage_rows(q, (char *)S->down[d], (char *)q->host.data(), false, &q->down_rows, budget);
+    if(q->down_rows < q->n1) { ++k; continue; }
+    S->down_owner[d] = nullptr;
+    q->down_slot = -1;
+    q->t_done = now;
+    q->active = false;
+    S->active.erase(S->active.begin() + k);
+    S->finished.push_back(q);
   }
   return FG2_OK;
 }
@@ -2164,17 +2225,18 @@ int fg2_server_submit_run(fg2_server *S, const fg2_run *like, int steps, const f
 int fg2_server_tick(fg2_server *S, double now, int *active, int *queued, int *finished)
 {
   if(!S) return fail(FG2_ERR_ARG, "NULL server");
-  if(int e = server_retire(S, now)) return e;
-  if(int e = server_admit(S, now)) return e;
+  size_t budget = fg2_server::COPY_BUDGET;
+  if(int e = server_retire(S, now, &budget)) return e;
+  if(int e = server_admit(S, now, &budget)) return e;
   if(int e = server_launch(S)) return e;
-  if(int e = server_retire(S, now)) return e;
-  if(S->active.empty() && S->queue.empty()) {
+  if(int e = server_retire(S, now, &budget)) return e;
+  if(S->active.empty() && S->queue.empty() && S->uploading.empty()) {
     for(int m = 0; m < module_count(); ++m) HIP_TRY(hipStreamSynchronize(S->mstream[m]));
     HIP_TRY(hipStreamSynchronize(S->copy));
     for(int j = 0; j < fg2_server::NPOOL; ++j) HIP_TRY(hipStreamSynchronize(S->pool[j]));
   }
   if(active) *active = (int)S->active.size();
-  if(queued) *queued = (int)S->queue.size();
+  if(queued) *queued = (int)(S->queue.size() + S->uploading.size());
   if(finished) *finished = (int)S->finished.size();
   return FG2_OK;
 }
@@ -2194,7 +2256,7 @@ int fg2_server_result(fg2_server *S, long long id, float *out, double *times, in
 int fg2_server_set_mode(fg2_server *S, int mode)
 {
   if(!S || (mode != FG2_SERVE_BATCHED && mode != FG2_SERVE_PER_RUN)) return fail(FG2_ERR_ARG, "bad server mode");
-  if(!S->active.empty() || !S->queue.empty())
+  if(!S->active.empty() || !S->queue.empty() || !S->uploading.empty())
     return fail(FG2_ERR_ARG, "the mode changes only while the server is idle");
   S->mode = mode;
   return FG2_OK;

@@ -348,8 +348,11 @@ int fg2_server_submit(fg2_server *server, const char *module, int n1, int n2, in
 int fg2_server_submit_run(fg2_server *server, const fg2_run *like, int steps, const float *init,
                           double now, long long *id);
 /* Retire finished runs, admit queued requests that fit the budget, and
- * launch one step of every active run.  Counts of active, queued and
- * finished requests are returned (pointers may be NULL). */
+ * launch one step of every active run.  Host copies of states into and out
+ * of the server's staging buffers are spread over ticks (at most 8 MiB per
+ * tick), so a large request takes several ticks to upload or to deliver.
+ * Counts of active, queued (waiting or uploading) and finished requests
+ * are returned (pointers may be NULL). */
 int fg2_server_tick(fg2_server *server, double now, int *active, int *queued, int *finished);
 /* times[3] = arrival, admission, completion (caller's clock; -1 if not yet);
  * *status = -1 while running, else FG2_OK / FG2_ERR_NAN; out (may be NULL)

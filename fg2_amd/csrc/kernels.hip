@@ -1,24 +1,24 @@
 // kernels.hip -- HIP kernels for gfx950: the plain per-operation path
-// (kick, drift, boundary fill, first_order) and the fused RK3 substage.
+// (kick, drift, boundary fill, first_order), the fused RK3 substage, the
+// one-pass RK3 step and their batched forms.
 //
 // Fused substage, one wavefront per tile of 64 columns x tile_h rows:
 //   - lane j owns column j0 + j; the wave marches down the slow axis (n1)
-//     keeping rows i-3 .. i+4 of its column in VGPRs (an 8-slot ring whose
-//     slot index is a compile-time constant in the 8x unrolled loop, so the
-//     ring never moves data);
-//   - row i+4 is loaded while row i is computed (one row of prefetch);
-//   - fast-axis neighbours j+-1..3 come from a wave-private LDS copy of row
-//     i widened by 3 halo cells per side; the halo cells of every loaded row
-//     are fetched by lanes 0..5 and kept in an 8-row LDS ring;
-//   - the mixed derivative uses g_k(x) = f(i+k,x) - f(i-k,x), staged in LDS,
-//     so D12 = sum_k m_k (g_k(j+k) - g_k(j-k));
+//     keeping rows i-3 .. i+3 of its column, and the next one or two rows
+//     in flight, in a register ring (8 or 9 slots) whose slot index is a
+//     compile-time constant in the unrolled loop, so the ring never moves
+//     data;
+//   - fast-axis neighbours j+-1..3 of the centre row come from ds_bpermute
+//     lane shifts (no LDS allocation, no barrier);
+//   - the mixed derivative uses g_k(x) = f(i+k,x) - f(i-k,x), shifted by
+//     +-k lanes, so D12 = sum_k m_k (g_k(j+k) - g_k(j-k));
 //   - boundaries are index maps (periodic wrap, even mirror on dim 1,
 //     mirror with the module's odd-variable sign on dim 2), which give the
 //     values the reference's ghost fill would (src/bcond.cu:22-86);
 //   - the epilogue does v <- F + beta v and u_out <- u_in + gamma dt v,
 //     skipping the v read in stage 0 (beta = 0) and the v write in stage 2
 //     (the next step's stage 0 ignores v).
-// Wavefronts of one workgroup never share LDS, so they synchronise only at
+// Wavefronts of one workgroup never share data, so they synchronise only at
 // wavefront scope: no s_barrier and no vmcnt drain inside the march.
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -108,14 +108,66 @@ __device__ __forceinline__ float lane_shift(float x, int addr)
   return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
 }
 
-template<class M, int STAGE>
+// A uniform pointer held in scalar registers: a per-lane 32-bit byte
+// offset from it then makes one global load or store with a scalar base and
+// a single offset register, instead of a 64-bit address pair per lane.
+template<class T> __device__ __forceinline__ T *uniform_ptr(T *p)
+{
+  const unsigned long long a = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  return (T *)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ const float *at_bytes(const float *p, unsigned b)
+{ return (const float *)((const char *)p + b); }
+__device__ __forceinline__ float *at_bytes(float *p, unsigned b)
+{ return (float *)((char *)p + b); }
+
+// Stencil primitives of one variable q of the centre row (ring window w at
+// offset p, horizontal neighbours by lane shifts) into x.
+template<class M, int RING>
+__device__ __forceinline__ void primitives(const float (&w)[RING][M::NV], const int p, const int q,
+                                           const int (&addr)[2][3], typename M::P_t &x)
+{
+  const float cq = w[(p + 3) % RING][q];
+#pragma unroll
+  for(int k = 1; k <= 3; ++k) {
+    const float a = w[(p + 3 + k) % RING][q], b = w[(p + 3 - k) % RING][q];
+    x.dv[k - 1][q] = a - b;
+    x.sv[k - 1][q] = a + b;
+    const float e = lane_shift(cq, addr[0][k - 1]);
+    const float f = lane_shift(cq, addr[1][k - 1]);
+    x.dh[k - 1][q] = e - f;
+    x.sh[k - 1][q] = e + f;
+  }
+}
+
+// The mixed-derivative differences of x (its dv already filled).
+template<class M>
+__device__ __forceinline__ void mixed(const int (&addr)[2][3], typename M::P_t &x)
+{
+#pragma unroll
+  for(int k = 1; k <= 3; ++k)
+#pragma unroll
+    for(int d = 0; d < M::ND; ++d) {
+      const float g = x.dv[k - 1][M::D12V[d]];    // f(i+k, .) - f(i-k, .)
+      x.mx[k - 1][d] = lane_shift(g, addr[0][k - 1]) - lane_shift(g, addr[1][k - 1]);
+    }
+}
+
+template<class M, int STAGE, bool CFL>
 __device__ __forceinline__ void process_tile(const float *__restrict__ uin, float *__restrict__ uout,
                                              float *__restrict__ v, const long s, const Params &P,
                                              const float beta, const float gdt,
                                              const int j0, const int i0, const int i1,
                                              const int cfl, TileOut &res)
 {
-  constexpr int NV = M::NV, ND = M::ND, RING = 9;
+  constexpr int NV = M::NV;
+  // u rows in the register ring: rows i-3..i+3 in use and the next one or
+  // two in flight.  A module whose right-hand side takes its derivatives
+  // first keeps one row in flight (8 slots), which with the derivatives in
+  // place of the primitives fits three wavefronts per SIMD (168 VGPRs).
+  constexpr int RING = M::DERIV_FIRST ? 8 : 9;
   const int lane = threadIdx.x & 63;
   const int n1 = P.n1, n2 = P.n2;
   // lane holds column c; lanes 3..60 with c < n2 produce output
@@ -123,7 +175,12 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
   const bool out = lane >= HALF && lane < 64 - HALF && c < n2;
   float sgn;
   const int cc = (c < n2 + HALF) ? map_col(c, n2, P.p2, sgn) : (sgn = 1.0f, n2 - 1);
-  const float *col = uin + (long)cc * NV;
+  // byte offset of the lane's column in a row: every load and store of the
+  // tile is a row's uniform address plus this register.  Lanes with output
+  // have cc == c; the others read a valid column, so that every load is
+  // unconditional (a predicated load would end in a register join that
+  // waits for it).
+  const unsigned coff = (unsigned)cc * NV * 4u;
   int addr[2][3];                      // bpermute addresses for lane +k / -k
 #pragma unroll
   for(int k = 1; k <= 3; ++k) {
@@ -132,25 +189,18 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
   }
   const float k1a = 1.5859783962656393f / P.Delta1, k1b = 1.5859783962656393f / P.Delta2;
 
-  // register ring: row r lives in slot (r - (i0 - 3)) % 9; rows i-3..i+3
-  // are in use, row i+4 is in flight, row i+5 is being requested
+  // register ring: row r lives in slot (r - (i0 - 3)) % RING; row i + RING
+  // - 4 is requested while row i is computed
   float w[RING][NV];
   // rows past i1 + 2 are never used; clamping keeps every load in bounds
   auto load_row = [&](int r, float (&dst)[NV]) {
-    load_cell<NV>(col + (long)map_row(min(r, i1 + 2), n1, P.p1) * s, dst);
+    load_cell<NV>(at_bytes(uniform_ptr(uin + (long)map_row(min(r, i1 + 2), n1, P.p1) * s), coff),
+                  dst);
   };
 #pragma unroll
   for(int k = 0; k < RING - 1; ++k) load_row(i0 - HALF + k, w[k]);
 #pragma unroll
-  for(int k = 0; k < RING - 2; ++k) w[k][M::ODD] *= sgn;
-
-  const long o0 = (long)(j0 - HALF + lane) * NV;   // output offset in a row
-  // v ring: row i lives in slot (i - i0) % 3; row i+1 is in flight.  Lanes
-  // without output read a valid column so that every load is unconditional
-  // (a predicated load would end in a register join that waits for it).
-  const float *vcol = v + (long)min(max(c, 0), n2 - 1) * NV;
-  float vr[3][NV];
-  if constexpr(STAGE > 0) load_cell<NV>(vcol + (long)i0 * s, vr[0]);
+  for(int k = 0; k < 2 * HALF + 1; ++k) w[k][M::ODD] *= sgn;
 
   for(int base = i0; base < i1; base += RING) {
 #pragma unroll
@@ -159,45 +209,45 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
       if(i >= i1) break;
       // row i+3 enters use: apply the reflective sign of this lane's column
       if(i > i0) w[(p + 6) % RING][M::ODD] *= sgn;
-      load_row(i + 5, w[(p + 8) % RING]);
-      if constexpr(STAGE > 0) load_cell<NV>(vcol + (long)min(i + 1, i1 - 1) * s, vr[(p + 1) % 3]);
-      const float (&vo)[NV] = vr[p % 3];
+      load_row(i + RING - 4, w[(p + RING - 1) % RING]);
+      // row i of v, requested as its iteration starts (the stencil and the
+      // right-hand side cover its latency)
+      float vo[NV];
+      if constexpr(STAGE > 0) load_cell<NV>(at_bytes(uniform_ptr(v + (long)i * s), coff), vo);
 
       const float (&cen)[NV] = w[(p + 3) % RING];
       typename M::P_t x;
 #pragma unroll
       for(int q = 0; q < NV; ++q) x.c[q] = cen[q];
-#pragma unroll
-      for(int k = 1; k <= 3; ++k) {
+      float f[NV];
+      if constexpr(M::DERIV_FIRST) {
+        // each variable's derivatives as soon as its stencil is read
+        typename M::Deriv D;
 #pragma unroll
         for(int q = 0; q < NV; ++q) {
-          const float a = w[(p + 3 + k) % RING][q], b = w[(p + 3 - k) % RING][q];
-          x.dv[k - 1][q] = a - b;
-          x.sv[k - 1][q] = a + b;
-          const float e = lane_shift(cen[q], addr[0][k - 1]);
-          const float f = lane_shift(cen[q], addr[1][k - 1]);
-          x.dh[k - 1][q] = e - f;
-          x.sh[k - 1][q] = e + f;
+          primitives<M, RING>(w, p, q, addr, x);
+          M::deriv_var(P, x, q, D);
         }
+        mixed<M>(addr, x);
+        M::deriv_mixed(P, x, D);
+        if(out) M::rhs_d(P, x.c, D, i, c, f);
+      } else {
 #pragma unroll
-        for(int d = 0; d < ND; ++d) {
-          const float g = x.dv[k - 1][M::D12V[d]];    // f(i+k, .) - f(i-k, .)
-          x.mx[k - 1][d] = lane_shift(g, addr[0][k - 1]) - lane_shift(g, addr[1][k - 1]);
-        }
+        for(int q = 0; q < NV; ++q) primitives<M, RING>(w, p, q, addr, x);
+        mixed<M>(addr, x);
+        if(out) M::rhs(P, x, i, c, f);
       }
       if(out) {
-        float f[NV], vn[NV], uo[NV];
-        M::rhs(P, x, i, c, f);
+        float vn[NV], uo[NV];
 #pragma unroll
         for(int q = 0; q < NV; ++q) {
           vn[q] = (STAGE == 0) ? f[q] : __builtin_fmaf(beta, vo[q], f[q]);
           uo[q] = __builtin_fmaf(gdt, vn[q], cen[q]);
           res.bad |= (uo[q] != uo[q]);
         }
-        const long o = (long)i * s + o0;
-        if constexpr(STAGE < 2) store_cell<NV>(v + o, vn);
-        store_cell<NV>(uout + o, uo);
-        if constexpr(STAGE == 2) {
+        if constexpr(STAGE < 2) store_cell<NV>(at_bytes(uniform_ptr(v + (long)i * s), coff), vn);
+        store_cell<NV>(at_bytes(uniform_ptr(uout + (long)i * s), coff), uo);
+        if constexpr(STAGE == 2 && CFL) {
           if(cfl) {
             res.lam_adv = fmaxf(res.lam_adv, M::lam_adv(P, uo, i, c, k1a, k1b));
             if constexpr(M::CELL_DIFF) res.lam_diff = fmaxf(res.lam_diff, M::lam_diff(P, uo, i, c));
@@ -231,8 +281,15 @@ __device__ __forceinline__ void commit_tile(const TileOut &r, const int cfl, Dev
   }
 }
 
-template<class M, int STAGE, int WPB>
-__global__ __launch_bounds__(64 * WPB)
+// Wavefronts per SIMD the substage kernel is compiled for: three (at most
+// 168 VGPRs) for a module whose right-hand side takes its derivatives first,
+// except in the stage-2 variant that reduces CFL rates.
+template<class M, bool CFL> constexpr int substage_waves() { return M::DERIV_FIRST && !CFL ? 3 : 1; }
+
+// CFL: the stage-2 variant that reduces the CFL rates of its output when
+// cfl is set (the others never do).
+template<class M, int STAGE, int WPB, bool CFL>
+__global__ __launch_bounds__(64 * WPB, (substage_waves<M, CFL>()))
 void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, float *__restrict__ v,
                      const long s, const Params P, const float beta, const float gdt,
                      const int tile_h, const int nstrips, const int ntiles, const int cfl,
@@ -244,8 +301,8 @@ void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, fl
   const int strip = tile % nstrips, chunk = tile / nstrips;
   const int i0 = chunk * tile_h, i1 = min(P.n1, i0 + tile_h);
   TileOut r{0, 0.0f, 0.0f};
-  process_tile<M, STAGE>(uin, uout, v, s, P, beta, gdt, strip * STRIP, i0, i1, cfl, r);
-  commit_tile(r, cfl, diag);
+  process_tile<M, STAGE, CFL>(uin, uout, v, s, P, beta, gdt, strip * STRIP, i0, i1, cfl, r);
+  commit_tile(r, CFL ? cfl : 0, diag);
 }
 
 // ---------------------------------------------------------------------------
@@ -693,7 +750,7 @@ void batch_kernel(const BatchRun *__restrict__ runs, const int nruns, const int 
     const int i0 = chunk * R.tile_h, i1 = min(R.P.n1, i0 + R.tile_h);
     TileOut res{0, 0.0f, 0.0f};
     const int cfl = STAGE == 2 ? R.cfl : 0;
-    process_tile<M, STAGE>(R.uin, R.uout, R.v, R.s, R.P, R.beta, R.gdt, strip * STRIP, i0, i1,
+    process_tile<M, STAGE, true>(R.uin, R.uout, R.v, R.s, R.P, R.beta, R.gdt, strip * STRIP, i0, i1,
                            cfl, res);
     commit_tile(res, cfl, R.diag);
   }
@@ -958,8 +1015,12 @@ static hipError_t launch_stage(const float *uin, float *uout, float *v, long s, 
   switch(sh.waves) {
 #define FG2_W(WPB)                                                              \
   case WPB:                                                                    \
-    substage_kernel<M, STAGE, WPB><<<(ntiles + WPB - 1) / WPB, 64 * WPB, 0, st>>>( \
-      uin, uout, v, s, P, beta, gdt, sh.tile_h, nstrips, ntiles, cfl, diag);    \
+    if(STAGE == 2 && cfl)                                                      \
+      substage_kernel<M, STAGE, WPB, STAGE == 2><<<(ntiles + WPB - 1) / WPB, 64 * WPB, 0, st>>>( \
+        uin, uout, v, s, P, beta, gdt, sh.tile_h, nstrips, ntiles, cfl, diag);  \
+    else                                                                       \
+      substage_kernel<M, STAGE, WPB, false><<<(ntiles + WPB - 1) / WPB, 64 * WPB, 0, st>>>( \
+        uin, uout, v, s, P, beta, gdt, sh.tile_h, nstrips, ntiles, 0, diag);    \
     break;
   FG2_W(1) FG2_W(2) FG2_W(4)
 #undef FG2_W
@@ -1060,9 +1121,9 @@ hipError_t kernel_occupancy(int module, int kind, int waves, int *blocks_per_cu)
     if(kind == 0 || kind == 2) {
 #define FG2_W(WPB)                                                                 \
       if(waves == WPB && kind == 0) {                                              \
-        take((const void *)substage_kernel<M, 0, WPB>);                            \
-        take((const void *)substage_kernel<M, 1, WPB>);                            \
-        take((const void *)substage_kernel<M, 2, WPB>);                            \
+        take((const void *)substage_kernel<M, 0, WPB, false>);                     \
+        take((const void *)substage_kernel<M, 1, WPB, false>);                     \
+        take((const void *)substage_kernel<M, 2, WPB, false>);                     \
       }                                                                            \
       if(waves == WPB && kind == 2) {                                              \
         take((const void *)batch_kernel<M, 0, WPB>);                               \

@@ -69,9 +69,44 @@ struct Disk {
   static __device__ __forceinline__ float4 row(const Params &P, int i)
   { return reinterpret_cast<const float4 *>(P.rowtab)[i]; }
 
+  // The right-hand side in two parts, so that a kernel can take each
+  // variable's derivatives as soon as its stencil is read (deriv_var) and
+  // keep only those, not the 90 stencil primitives, until rhs_d runs: D1
+  // and D2 of every variable, D11 and D22 of all but Omg and lne, whose
+  // viscous terms need only their Laplacian, and D12 of the D12V variables.
+  static constexpr bool DERIV_FIRST = true;
+  enum { LND, UR, UZ, OMG, LNE, LNER };
+  struct Deriv { float a[NV], b[NV], d11[NV], d22[NV], lap[NV], d12[ND]; };
+  static __device__ __forceinline__ void deriv_var(const Params &P, const P_t &x, int q, Deriv &D)
+  {
+    D.a[q] = D1(P, x, q);
+    D.b[q] = D2(P, x, q);
+    if(q == OMG || q == LNE) {
+      D.lap[q] = LAP(P, x, q);
+    } else {
+      D.d11[q] = D11(P, x, q);
+      D.d22[q] = D22(P, x, q);
+    }
+  }
+  static __device__ __forceinline__ void deriv_mixed(const Params &P, const P_t &x, Deriv &D)
+  {
+#pragma unroll
+    for(int d = 0; d < ND; ++d) D.d12[d] = D12(P, x, d);
+  }
+
   static __device__ __forceinline__ void rhs(const Params &P, const P_t &x, int i, int j, float dt[NV])
   {
-    enum { LND, UR, UZ, OMG, LNE, LNER };
+    Deriv D;
+#pragma unroll
+    for(int q = 0; q < NV; ++q) deriv_var(P, x, q, D);
+    deriv_mixed(P, x, D);
+    rhs_d(P, x.c, D, i, j, dt);
+  }
+
+  // The right-hand side from the centre values xc and the derivatives D.
+  static __device__ __forceinline__ void rhs_d(const Params &P, const float (&xc)[NV], const Deriv &D,
+                                               int i, int j, float dt[NV])
+  {
     const float M = P.par[0], g = P.par[3];
     const float nus = P.par[4], nub = P.par[5], kap = P.par[6], alpha = P.par[7];
     const float ar = P.par[8], es = P.par[9], ff = P.par[10];
@@ -87,7 +122,7 @@ struct Disk {
     const float r = sphr * sint;
     const float ir = isphr * __builtin_amdgcn_rcpf(sint);
 
-    const float den = expf(x.c[LND]), eng = expf(x.c[LNE]), rad = expf(x.c[LNER]);
+    const float den = expf(xc[LND]), eng = expf(xc[LNE]), rad = expf(xc[LNER]);
     const float ieng = __builtin_amdgcn_rcpf(eng);
     const float gamma1 = g - 1.0f;
     const float temp = gamma1 * eng;
@@ -99,22 +134,22 @@ struct Disk {
     float dr[NV], dz[NV], lap[NV], d11[NV], d22[NV];
 #pragma unroll
     for(int q = 0; q < NV; ++q) {
-      const float a = D1(P, x, q), b = D2(P, x, q);
+      const float a = D.a[q], b = D.b[q];
       dr[q] = sr * a + cr * b;
       dz[q] = cr * a - sr * b;
       if(q != OMG && q != LNE) {          // Omg and lne need only their Laplacian
-        d11[q] = D11(P, x, q);
-        d22[q] = D22(P, x, q);
+        d11[q] = D.d11[q];
+        d22[q] = D.d22[q];
       }
     }
-    const float d12_lnd = D12(P, x, 0), d12_ur = D12(P, x, 1);
-    const float d12_uz = D12(P, x, 2), d12_lner = D12(P, x, 3);
+    const float d12_lnd = D.d12[0], d12_ur = D.d12[1];
+    const float d12_uz = D.d12[2], d12_lner = D.d12[3];
     const float cc = cr * cr, cs = cr * sr, ss = sr * sr;
     const float c2 = cc - ss, s2 = 2.0f * cs;
     const float isphr_2 = isphr * isphr;
 #pragma unroll
     for(int q = 0; q < NV; ++q)
-      lap[q] = (q == OMG || q == LNE ? LAP(P, x, q) : d11[q] + d22[q]) * isphr_2;
+      lap[q] = (q == OMG || q == LNE ? D.lap[q] : d11[q] + d22[q]) * isphr_2;
 
     const float drr_lnd  = ss * d11[LND]  + cc * d22[LND]  + s2 * d12_lnd  + cr * dz[LND]  - sr * dr[LND];
     const float drr_ur   = ss * d11[UR]   + cc * d22[UR]   + s2 * d12_ur   + cr * dz[UR]   - sr * dr[UR];
@@ -127,7 +162,7 @@ struct Disk {
     const float dzz_uz   = cc * d11[UZ]   + ss * d22[UZ]   - s2 * d12_uz   - cr * dz[UZ]   + sr * dr[UZ];
     const float dzz_lner = cc * d11[LNER] + ss * d22[LNER] - s2 * d12_lner - cr * dz[LNER] + sr * dr[LNER];
 
-    const float ur = x.c[UR], uz = x.c[UZ], Omg = x.c[OMG];
+    const float ur = xc[UR], uz = xc[UZ], Omg = xc[OMG];
     dt[LND]  = -(ur * dr[LND]  + uz * dz[LND]);                          // advection, pseudo-force
     dt[UR]   = -(ur * dr[UR]   + uz * dz[UR] - Omg * Omg * r);
     dt[UZ]   = -(ur * dr[UZ]   + uz * dz[UZ]);

@@ -13,6 +13,8 @@ struct Hydro {
   static constexpr int D12V[ND] = {1, 2};
   static constexpr bool ONE_PASS = true, ROW_TABLE = false, HAS_FIRST_ORDER = false,
                         RANDOM_PHASES = false, CELL_DIFF = false;
+  // the right-hand side takes the stencil primitives whole (rhs)
+  static constexpr bool DERIV_FIRST = false;
   using P_t = Prim<NV, ND>;
 
   // par: gamma nus nub kappa dd (src/hydro/eqns.cuh:19-24)

@@ -14,6 +14,8 @@ struct HydroIsot {
   static constexpr int D12V[ND] = {1, 2};
   static constexpr bool ONE_PASS = true, ROW_TABLE = false, HAS_FIRST_ORDER = true,
                         RANDOM_PHASES = true, CELL_DIFF = false;
+  // the right-hand side takes the stencil primitives whole (rhs)
+  static constexpr bool DERIV_FIRST = false;
   using P_t = Prim<NV, ND>;
 
   // par: dd nus nub cs2 f k (src/hydro-isot/eqns.cuh:19-25)

@@ -16,6 +16,11 @@
 //   ONE_PASS      whether the one-pass step kernel applies (register rings fit)
 //   ROW_TABLE     whether it reads a per-row table (row_entry(), Params::rowtab)
 //   rhs()         the module's eqns()                               (eqns.cuh)
+//   DERIV_FIRST   whether rhs() also comes in two parts, deriv_var() /
+//                 deriv_mixed() (the derivatives of one variable, the
+//                 mixed ones) and rhs_d() (the rest, from those), so that
+//                 the substage kernel keeps derivatives, not stencil
+//                 primitives, live through the right-hand side
 //   first_order() and first_order_on(), HAS_FIRST_ORDER, RANDOM_PHASES
 //                 the point-wise pre-step, when it runs, and whether it
 //                 draws two random phases per step                  (Euler.cuh)

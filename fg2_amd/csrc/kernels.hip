@@ -201,6 +201,14 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
   for(int k = 0; k < RING - 1; ++k) load_row(i0 - HALF + k, w[k]);
 #pragma unroll
   for(int k = 0; k < 2 * HALF + 1; ++k) w[k][M::ODD] *= sgn;
+  // v rows: a module whose right-hand side takes its derivatives first
+  // requests row i's v as its iteration starts (the stencil and the
+  // right-hand side cover the latency; one slot), the others keep a 3-slot
+  // ring with row i+1 in flight (their substage kernels are HBM bound)
+  constexpr int VR = M::DERIV_FIRST ? 1 : 3;
+  static_assert(RING % VR == 0, "the v ring's slot is the iteration modulo VR");
+  float vr[VR][NV];
+  if constexpr(STAGE > 0 && VR > 1) load_cell<NV>(at_bytes(uniform_ptr(v + (long)i0 * s), coff), vr[0]);
 
   for(int base = i0; base < i1; base += RING) {
 #pragma unroll
@@ -210,10 +218,14 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
       // row i+3 enters use: apply the reflective sign of this lane's column
       if(i > i0) w[(p + 6) % RING][M::ODD] *= sgn;
       load_row(i + RING - 4, w[(p + RING - 1) % RING]);
-      // row i of v, requested as its iteration starts (the stencil and the
-      // right-hand side cover its latency)
-      float vo[NV];
-      if constexpr(STAGE > 0) load_cell<NV>(at_bytes(uniform_ptr(v + (long)i * s), coff), vo);
+      if constexpr(STAGE > 0) {
+        if constexpr(VR == 1)
+          load_cell<NV>(at_bytes(uniform_ptr(v + (long)i * s), coff), vr[0]);
+        else
+          load_cell<NV>(at_bytes(uniform_ptr(v + (long)min(i + 1, i1 - 1) * s), coff),
+                        vr[(p + 1) % VR]);
+      }
+      const float (&vo)[NV] = vr[p % VR];
 
       const float (&cen)[NV] = w[(p + 3) % RING];
       typename M::P_t x;

@@ -8,8 +8,9 @@
 //     in flight, in a register ring (8 or 9 slots) whose slot index is a
 //     compile-time constant in the unrolled loop, so the ring never moves
 //     data;
-//   - fast-axis neighbours j+-1..3 of the centre row come from ds_bpermute
-//     lane shifts (no LDS allocation, no barrier);
+//   - fast-axis neighbours of the centre row come from DPP wave shifts
+//     (j+-1) and ds_bpermute lane shifts (j+-2, j+-3): no LDS allocation,
+//     no barrier;
 //   - the mixed derivative uses g_k(x) = f(i+k,x) - f(i-k,x), shifted by
 //     +-k lanes, so D12 = sum_k m_k (g_k(j+k) - g_k(j-k));
 //   - boundaries are index maps (periodic wrap, even mirror on dim 1,
@@ -123,8 +124,17 @@ __device__ __forceinline__ const float *at_bytes(const float *p, unsigned b)
 __device__ __forceinline__ float *at_bytes(float *p, unsigned b)
 { return (float *)((char *)p + b); }
 
+// Value of x at lane (lane + 1) / (lane - 1) of the wavefront, by a DPP
+// wave shift (a VALU move, no LDS round trip; lanes 63 / 0 read zero, and
+// produce no output).
+__device__ __forceinline__ float lane_next(float x)
+{ return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false)); }
+__device__ __forceinline__ float lane_prev(float x)
+{ return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false)); }
+
 // Stencil primitives of one variable q of the centre row (ring window w at
-// offset p, horizontal neighbours by lane shifts) into x.
+// offset p) into x.  Horizontal neighbours: +-1 by DPP wave shifts, +-2 and
+// +-3 by ds_bpermute.
 template<class M, int RING>
 __device__ __forceinline__ void primitives(const float (&w)[RING][M::NV], const int p, const int q,
                                            const int (&addr)[2][3], typename M::P_t &x)
@@ -135,8 +145,8 @@ __device__ __forceinline__ void primitives(const float (&w)[RING][M::NV], const 
     const float a = w[(p + 3 + k) % RING][q], b = w[(p + 3 - k) % RING][q];
     x.dv[k - 1][q] = a - b;
     x.sv[k - 1][q] = a + b;
-    const float e = lane_shift(cq, addr[0][k - 1]);
-    const float f = lane_shift(cq, addr[1][k - 1]);
+    const float e = k == 1 ? lane_next(cq) : lane_shift(cq, addr[0][k - 1]);
+    const float f = k == 1 ? lane_prev(cq) : lane_shift(cq, addr[1][k - 1]);
     x.dh[k - 1][q] = e - f;
     x.sh[k - 1][q] = e + f;
   }
@@ -151,7 +161,8 @@ __device__ __forceinline__ void mixed(const int (&addr)[2][3], typename M::P_t &
 #pragma unroll
     for(int d = 0; d < M::ND; ++d) {
       const float g = x.dv[k - 1][M::D12V[d]];    // f(i+k, .) - f(i-k, .)
-      x.mx[k - 1][d] = lane_shift(g, addr[0][k - 1]) - lane_shift(g, addr[1][k - 1]);
+      x.mx[k - 1][d] = k == 1 ? lane_next(g) - lane_prev(g)
+                              : lane_shift(g, addr[0][k - 1]) - lane_shift(g, addr[1][k - 1]);
     }
 }
 

@@ -269,7 +269,7 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_
            "wall_s": wall, "ticks": ticks, "host_s_in_tick": tick_s,
            "host_frac_of_wall": tick_s / wall,
            "hbm_budget_gib": budget_gib,
-           "waves": waves if waves > 0 else "chosen per launch (rounds of resident workgroups)",
+           "waves": waves if waves > 0 else 2,
            "tile_h": tile_h if tile_h > 0 else "chosen per launch (rounds of resident workgroups)",
            "nan_requests": bad}
     zero = np.zeros(ncap)

@@ -328,9 +328,10 @@ typedef struct fg2_server fg2_server;
  * and advances all active runs together each tick (one batched one-pass
  * launch per module, after the batched Euler pre-step of the runs that have
  * one; served runs are periodic hydro or hydro-isot).  waves, tile_h: the
- * batched kernel's
- * wavefronts per workgroup and tile height; 0 chooses it per launch from
- * the active runs (rounds of resident workgroups times the tile's cost). */
+ * batched kernel's wavefronts per workgroup and tile height; waves = 0
+ * takes two-wavefront workgroups and tile_h = 0 chooses the height per
+ * launch from the active runs (rounds of resident workgroups times the
+ * tile's cost). */
 int fg2_server_create(fg2_server **server, size_t hbm_budget, int waves, int tile_h);
 void fg2_server_free(fg2_server *server);
 /* Queue a request of `steps` RK3 steps (closed-form dt per step) from the

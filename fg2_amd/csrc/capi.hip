@@ -1775,12 +1775,15 @@ static_assert(Modules::size <= fg2_server::MAXM, "one compute stream per module"
 
 namespace {
 
+// Grow a pinned staging buffer to hold `need` bytes: at least double its
+// size, so a stream of growing requests reallocates (a slow pinned
+// allocation that holds up the tick) a few times, not at every new maximum.
 int stage_buffer(float **buf, size_t *cap, size_t need)
 {
   if(need <= *cap) return FG2_OK;
   if(*buf) HIP_TRY(hipHostFree(*buf));
   *buf = nullptr;
-  const size_t c = std::max(need, (size_t)1 << 24);
+  const size_t c = std::max({need, 2 * *cap, (size_t)1 << 24});
   HIP_TRY(hipHostMalloc((void **)buf, c, hipHostMallocDefault));
   *cap = c;
   return FG2_OK;

@@ -109,20 +109,58 @@ __device__ __forceinline__ float lane_shift(float x, int addr)
   return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
 }
 
-// A uniform pointer held in scalar registers: a per-lane 32-bit byte
-// offset from it then makes one global load or store with a scalar base and
-// a single offset register, instead of a 64-bit address pair per lane.
-template<class T> __device__ __forceinline__ T *uniform_ptr(T *p)
+// Global-memory (address space 1) byte pointers.  A uniform row address
+// held in scalar registers plus a per-lane 32-bit byte offset makes one
+// global load or store with a scalar base and a single offset register
+// (global_load ... vN, s[..]), instead of a 64-bit address pair per lane;
+// keeping the global address space keeps these loads off the LDS wait
+// counter (a flat access counts on both).
+#define FG2_GLOBAL __attribute__((address_space(1)))
+using gbytes = FG2_GLOBAL char *;
+using gcbytes = FG2_GLOBAL const char *;
+
+// p (the same in every lane) as a global byte pointer in scalar registers
+__device__ __forceinline__ unsigned long long uniform_addr(const void *p)
 {
   const unsigned long long a = (unsigned long long)p;
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
   const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-  return (T *)(((unsigned long long)hi << 32) | lo);
+  return ((unsigned long long)hi << 32) | lo;
 }
-__device__ __forceinline__ const float *at_bytes(const float *p, unsigned b)
-{ return (const float *)((const char *)p + b); }
-__device__ __forceinline__ float *at_bytes(float *p, unsigned b)
-{ return (float *)((char *)p + b); }
+__device__ __forceinline__ gcbytes uniform_gptr(const float *p) { return (gcbytes)uniform_addr(p); }
+__device__ __forceinline__ gbytes uniform_gptr(float *p) { return (gbytes)uniform_addr(p); }
+
+// One cell (NV consecutive floats) at a global byte address, as load_cell
+// and store_cell do it at a generic one.
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+template<int NV> __device__ __forceinline__ void gload_cell(gcbytes p, float (&x)[NV])
+{
+  if constexpr(NV == 4) {
+    const fvec4 a = *(FG2_GLOBAL const fvec4 *)p;
+    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+  } else if constexpr(NV == 6) {
+    const fvec2 a = *(FG2_GLOBAL const fvec2 *)p, b = *(FG2_GLOBAL const fvec2 *)(p + 8),
+                c = *(FG2_GLOBAL const fvec2 *)(p + 16);
+    x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y; x[4] = c.x; x[5] = c.y;
+  } else {
+#pragma unroll
+    for(int q = 0; q < NV; ++q) x[q] = ((FG2_GLOBAL const float *)p)[q];
+  }
+}
+template<int NV> __device__ __forceinline__ void gstore_cell(gbytes p, const float (&x)[NV])
+{
+  if constexpr(NV == 4) {
+    *(FG2_GLOBAL fvec4 *)p = fvec4{x[0], x[1], x[2], x[3]};
+  } else if constexpr(NV == 6) {
+    *(FG2_GLOBAL fvec2 *)p = fvec2{x[0], x[1]};
+    *(FG2_GLOBAL fvec2 *)(p + 8) = fvec2{x[2], x[3]};
+    *(FG2_GLOBAL fvec2 *)(p + 16) = fvec2{x[4], x[5]};
+  } else {
+#pragma unroll
+    for(int q = 0; q < NV; ++q) ((FG2_GLOBAL float *)p)[q] = x[q];
+  }
+}
 
 // Value of x at lane (lane + 1) / (lane - 1) of the wavefront, by a DPP
 // wave shift (a VALU move, no LDS round trip; lanes 63 / 0 read zero, and
@@ -205,8 +243,7 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
   float w[RING][NV];
   // rows past i1 + 2 are never used; clamping keeps every load in bounds
   auto load_row = [&](int r, float (&dst)[NV]) {
-    load_cell<NV>(at_bytes(uniform_ptr(uin + (long)map_row(min(r, i1 + 2), n1, P.p1) * s), coff),
-                  dst);
+    gload_cell<NV>(uniform_gptr(uin + (long)map_row(min(r, i1 + 2), n1, P.p1) * s) + coff, dst);
   };
 #pragma unroll
   for(int k = 0; k < RING - 1; ++k) load_row(i0 - HALF + k, w[k]);
@@ -219,7 +256,7 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
   constexpr int VR = M::DERIV_FIRST ? 1 : 3;
   static_assert(RING % VR == 0, "the v ring's slot is the iteration modulo VR");
   float vr[VR][NV];
-  if constexpr(STAGE > 0 && VR > 1) load_cell<NV>(at_bytes(uniform_ptr(v + (long)i0 * s), coff), vr[0]);
+  if constexpr(STAGE > 0 && VR > 1) gload_cell<NV>(uniform_gptr(v + (long)i0 * s) + coff, vr[0]);
 
   for(int base = i0; base < i1; base += RING) {
 #pragma unroll
@@ -228,14 +265,13 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
       if(i >= i1) break;
       // row i+3 enters use: apply the reflective sign of this lane's column
       if(i > i0) w[(p + 6) % RING][M::ODD] *= sgn;
+      // loads complete in issue order for the wait counter: row i's v goes
+      // first, so waiting for it leaves the u row requested next in flight
+      if constexpr(STAGE > 0 && VR == 1)
+        gload_cell<NV>(uniform_gptr(v + (long)i * s) + coff, vr[0]);
       load_row(i + RING - 4, w[(p + RING - 1) % RING]);
-      if constexpr(STAGE > 0) {
-        if constexpr(VR == 1)
-          load_cell<NV>(at_bytes(uniform_ptr(v + (long)i * s), coff), vr[0]);
-        else
-          load_cell<NV>(at_bytes(uniform_ptr(v + (long)min(i + 1, i1 - 1) * s), coff),
-                        vr[(p + 1) % VR]);
-      }
+      if constexpr(STAGE > 0 && VR > 1)
+        gload_cell<NV>(uniform_gptr(v + (long)min(i + 1, i1 - 1) * s) + coff, vr[(p + 1) % VR]);
       const float (&vo)[NV] = vr[p % VR];
 
       const float (&cen)[NV] = w[(p + 3) % RING];
@@ -268,8 +304,8 @@ __device__ __forceinline__ void process_tile(const float *__restrict__ uin, floa
           uo[q] = __builtin_fmaf(gdt, vn[q], cen[q]);
           res.bad |= (uo[q] != uo[q]);
         }
-        if constexpr(STAGE < 2) store_cell<NV>(at_bytes(uniform_ptr(v + (long)i * s), coff), vn);
-        store_cell<NV>(at_bytes(uniform_ptr(uout + (long)i * s), coff), uo);
+        if constexpr(STAGE < 2) gstore_cell<NV>(uniform_gptr(v + (long)i * s) + coff, vn);
+        gstore_cell<NV>(uniform_gptr(uout + (long)i * s) + coff, uo);
         if constexpr(STAGE == 2 && CFL) {
           if(cfl) {
             res.lam_adv = fmaxf(res.lam_adv, M::lam_adv(P, uo, i, c, k1a, k1b));

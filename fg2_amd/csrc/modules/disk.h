@@ -67,7 +67,11 @@ struct Disk {
   // Row quantities of the (log r, theta) grid, from the run's table (one
   // scalar load: the row index is uniform across a wavefront).
   static __device__ __forceinline__ float4 row(const Params &P, int i)
-  { return reinterpret_cast<const float4 *>(P.rowtab)[i]; }
+  {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 r = ((__attribute__((address_space(4))) const v4 *)P.rowtab)[i];
+    return make_float4(r.x, r.y, r.z, r.w);
+  }
 
   // The right-hand side in two parts, so that a kernel can take each
   // variable's derivatives as soon as its stencil is read (deriv_var) and

@@ -162,6 +162,13 @@ template<int NV> __device__ __forceinline__ void gstore_cell(gbytes p, const flo
   }
 }
 
+// A pointer known to point into global memory, as an address-space-1
+// pointer: loads and stores through it are global (on the vector-memory
+// wait counter only) even where the compiler cannot see where the pointer
+// came from (a batch's run records hold the runs' pointers).
+__device__ __forceinline__ gcbytes gptr(const float *p) { return (gcbytes)p; }
+__device__ __forceinline__ gbytes gptr(float *p) { return (gbytes)p; }
+
 // Value of x at lane (lane + 1) / (lane - 1) of the wavefront, by a DPP
 // wave shift (a VALU move, no LDS round trip; lanes 63 / 0 read zero, and
 // produce no output).
@@ -515,11 +522,11 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
       // which no stage reads, so it is clamped into the buffer
       const float *base = r < 0 ? H.lo : r >= n1 ? H.hi : uin;
       const int rr = r < 0 ? r + SLAB_HALO : r >= n1 ? min(r - n1, SLAB_HALO - 1) : r;
-      load_cell<NV>(base + (long)rr * s + (long)cc * NV, dst);
+      gload_cell<NV>(gptr(base + (long)rr * s + (long)cc * NV), dst);
     } else if constexpr(BOUNDED) {
-      load_cell<NV>(col + (long)(mirror_rows ? mirror(r, n1) : pmod(r, n1)) * s, dst);
+      gload_cell<NV>(gptr(col + (long)(mirror_rows ? mirror(r, n1) : pmod(r, n1)) * s), dst);
     } else {
-      load_cell<NV>(col + (long)pmod(r, n1) * s, dst);
+      gload_cell<NV>(gptr(col + (long)pmod(r, n1) * s), dst);
     }
   };
 #pragma unroll
@@ -636,7 +643,7 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
         if(out && r2 < i1) {
 #pragma unroll
           for(int q = 0; q < NV; ++q) res.bad |= (uo[q] != uo[q]);
-          store_cell<NV>(uout + (long)r2 * s + (long)c * NV, uo);
+          gstore_cell<NV>(gptr(uout + (long)r2 * s + (long)c * NV), uo);
           if(cfl) {
             res.lam_adv = fmaxf(res.lam_adv, M::lam_adv(P, uo, r2, c, k1a, k1b));
             if constexpr(M::CELL_DIFF) res.lam_diff = fmaxf(res.lam_diff, M::lam_diff(P, uo, r2, c));

@@ -332,17 +332,25 @@ __device__ __forceinline__ float wave_max(float x)
 }
 
 // Fold one tile's NaN flag and CFL rates into the run's diagnostics.
+// (device-scope relaxed atomics, as atomicOr / atomicMax, through global
+// pointers: the diagnostics live in device memory)
 __device__ __forceinline__ void commit_tile(const TileOut &r, const int cfl, DeviceDiag *diag)
 {
   const unsigned long long badmask = __ballot(r.bad);
   const int lane = threadIdx.x & 63;
-  if(badmask && lane == 0) atomicOr(&diag->flags, 1u);
+  if(badmask && lane == 0)
+    __hip_atomic_fetch_or((FG2_GLOBAL unsigned int *)&diag->flags, 1u, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
   if(cfl) {
     const float la = wave_max(r.lam_adv), ld = wave_max(r.lam_diff);
     if(lane == 0) {
       // non-negative doubles order like their bit patterns
-      atomicMax(&diag->lam_adv_bits, (unsigned long long)__double_as_longlong((double)la));
-      atomicMax(&diag->lam_diff_bits, (unsigned long long)__double_as_longlong((double)ld));
+      __hip_atomic_fetch_max((FG2_GLOBAL unsigned long long *)&diag->lam_adv_bits,
+                             (unsigned long long)__double_as_longlong((double)la),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_max((FG2_GLOBAL unsigned long long *)&diag->lam_diff_bits,
+                             (unsigned long long)__double_as_longlong((double)ld),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }

@@ -1,7 +1,7 @@
 """numpy restatement of the reference's IDL spectral diagnostics (FP64).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/ and bench.py's checks may import this
-module; it is the checker of fg2_spectrum / fg2_vorticity, never part of the
+module; it is what fg2_spectrum / fg2_vorticity are compared with, never part of the
 product.
 
     getk(n1, n2, zeronyquist)   tools/getk.pro:18-33

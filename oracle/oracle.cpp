@@ -1,6 +1,6 @@
 // oracle.cpp -- host restatement of fg2's time-step path.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+// TEST INFRASTRUCTURE ONLY.  This file is what GPU results are compared with: only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load the
 // library built from it.  The product (fg2_amd/) never links or calls it.
 //
@@ -729,7 +729,7 @@ template<class R> void init_ic(Sim<R> &S, const std::string &name, uint64_t seed
 } // namespace
 
 // ---------------------------------------------------------------------------
-// C ABI for the ctypes checker (oracle/oracle.py)
+// C ABI for the ctypes wrapper (oracle/oracle.py)
 // ---------------------------------------------------------------------------
 struct OrcHandle { int fp64; void *sim; };
 

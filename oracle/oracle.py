@@ -1,8 +1,8 @@
 """ctypes wrapper of the host oracle (oracle/oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg may import this module; it is the checker and the
-CPU baseline, never the thing measured or shipped.  The product package
+bench.py's cpu_baseline leg may import this module; it is what GPU results are compared with and
+the CPU baseline, never the thing measured or shipped.  The product package
 (fg2_amd) does not import it.
 
 The oracle restates the reference's time-step path on the host, templated on

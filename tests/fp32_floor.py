@@ -17,7 +17,7 @@ sequence as the GPU parity tests:
         batched workload's size range), closed-form dt, 100 steps.
 
 Reported per field: relative L2 (normalised by the FP64 field) of FP32 vs
-FP64, and the same for the velocity vector.  The oracle is the checker here;
+FP64, and the same for the velocity vector.  The oracle is the reference here;
 nothing in the product imports this script.
 """
 import argparse

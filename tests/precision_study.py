@@ -15,7 +15,7 @@ and Linf error of the FP32 GPU state against the FP64 oracle, the relative
 drift of total mass and of angular momentum sum(rho Omega R^2 dV) on each
 side, and the wall time of each side.
 
-The oracle is the checker here: this script compares against it and is not
+The oracle is the reference here: this script compares against it and is not
 part of the product path.
 """
 import argparse

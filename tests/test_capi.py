@@ -49,7 +49,7 @@ def test_error_paths_without_device():
 
 
 def test_product_does_not_import_the_oracle():
-    """The product package never routes through the checker."""
+    """The product package never routes through the oracle."""
     for dirpath, _, files in os.walk(os.path.join(ROOT, "fg2_amd")):
         for f in files:
             if f.endswith((".py", ".hip", ".h", ".cpp")):

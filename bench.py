@@ -244,7 +244,7 @@ def _serve(arrivals, mods, n1s, n2s, steps, states, budget_gib, waves=0, tile_h=
 
 
 def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_h=0,
-                  ncap=300):
+                  ncap=300, nplain=100):
     """Poisson stream of run requests (rate per second) mixing hydro and
     hydro-isot, sides log-uniform in 64..2048, steps uniform in 50..2000,
     seeded random-field initial states (fg2_amd.ics.serving_workload);
@@ -253,7 +253,8 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_
     the offered load), then the server's capacity: the first `ncap` requests
     all arriving at once (the queue never drains until the end), served
     batched and served per run (each run's own one-pass launch per step on
-    8 streams, same admission and arena), and their ratio."""
+    8 streams, same admission and arena), and their ratio; and on the first
+    `nplain` requests the same against each run's plain kernels."""
     from fg2_amd import ics
     arrivals, mods, n1s, n2s, steps, make = ics.serving_workload(nreq, rate, seed)
     # initial states are generated before the timed loop (host set-up)
@@ -282,6 +283,17 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_
         cap[name] = {"cell_updates_per_s": cell_steps / wall, "wall_s": wall, "ticks": ticks,
                      "host_s_in_tick": tick_s, "p50_s": float(np.percentile(lat, 50)),
                      "p99_s": float(np.percentile(lat, 99)), "nan_requests": bad}
+    # the plain-kernel baseline is 8-10x slower: a shorter saturated queue
+    # (the first nplain requests), measured batched and per run alike
+    plain = {}
+    for name, mode in (("batched", 0), ("per_run_plain", 2)):
+        z = np.zeros(nplain)
+        lat, bad, cell_steps, wall, ticks, tick_s = _serve(z, mods[:nplain], n1s[:nplain],
+                                                          n2s[:nplain], steps[:nplain],
+                                                          states[:nplain], budget_gib, waves,
+                                                          tile_h, mode)
+        plain[name] = {"cell_updates_per_s": cell_steps / wall, "wall_s": wall, "ticks": ticks,
+                       "nan_requests": bad}
     out["capacity_cell_updates_per_s"] = cap["batched"]["cell_updates_per_s"]
     out["capacity"] = {"workload": f"the first {ncap} requests of the stream, all arriving "
                                    "at t = 0 (a saturated queue)",
@@ -291,7 +303,17 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_
                        "per_run_note": "each admitted run launches its own one-pass step "
                                        "kernel (the shape fg2_step gives a run of its "
                                        "size) per tick, on 8 streams; same admission, "
-                                       "arena and copies"}
+                                       "arena and copies",
+                       "plain": {"workload": f"the first {nplain} requests of the stream, all "
+                                             "arriving at t = 0",
+                                 "batched": plain["batched"],
+                                 "per_run_plain": plain["per_run_plain"],
+                                 "speedup_vs_per_run_plain": plain["batched"]["cell_updates_per_s"] /
+                                                             plain["per_run_plain"]["cell_updates_per_s"],
+                                 "note": "per_run_plain: each admitted run launches the plain "
+                                         "kernels (bcond, kick, drift per substage, 9 per "
+                                         "step) per tick on 8 streams; same admission, arena "
+                                         "and copies"}}
     return out
 
 

@@ -78,7 +78,7 @@ FIELDS = _Modules(lambda nv, fields: fields)    # module name -> state variable 
 OK, ERR_NAN, ERR_HIP, ERR_SHAPE, ERR_ARG, ERR_NOMEM = range(6)
 PATH_FUSED, PATH_PLAIN, PATH_SUBSTAGE = 0, 1, 2
 DT_CLOSED, DT_CFL = 0, 1
-SERVE_BATCHED, SERVE_PER_RUN = 0, 1
+SERVE_BATCHED, SERVE_PER_RUN, SERVE_PER_RUN_PLAIN = 0, 1, 2
 
 
 class Fg2Error(RuntimeError):
@@ -575,8 +575,9 @@ class Server:
         return st.value, tuple(t.tolist()), (out if st.value >= 0 else None)
 
     def set_mode(self, mode):
-        """SERVE_BATCHED (one batched launch per module per tick) or
-        SERVE_PER_RUN (each run's own step launch, the baseline)."""
+        """SERVE_BATCHED (one batched launch per module per tick),
+        SERVE_PER_RUN (each run's own step launch) or SERVE_PER_RUN_PLAIN
+        (each run's plain kernels); the per-run modes are the baselines."""
         _check(lib.fg2_server_set_mode(self.h, int(mode)))
 
     def stats(self):

@@ -1839,7 +1839,7 @@ int server_grow(fg2_server *S, size_t need)
 // The stream a request's steps run on.
 hipStream_t run_stream(const fg2_server *S, const Request *q)
 {
-  return S->mode == FG2_SERVE_PER_RUN ? S->pool[q->sidx] : S->mstream[q->module];
+  return S->mode != FG2_SERVE_BATCHED ? S->pool[q->sidx] : S->mstream[q->module];
 }
 
 // Bytes of one row of a request's state in its host array.
@@ -1916,10 +1916,15 @@ int server_admit(fg2_server *S, double now, size_t *budget)
     HIP_TRY(hipEventRecord(S->up_done[u], S->copy));
     S->up_owner[u] = nullptr;
     q->up_slot = -1;
-    if(S->mode == FG2_SERVE_PER_RUN) {
+    if(S->mode != FG2_SERVE_BATCHED) {
       q->sidx = S->next_sidx;
       S->next_sidx = (S->next_sidx + 1) % fg2_server::NPOOL;
     }
+    // the plain path's first kick multiplies the old v by beta_0 = 0: start
+    // v at zero so that whatever the arena held cannot make it NaN
+    if(S->mode == FG2_SERVE_PER_RUN_PLAIN)
+      HIP_TRY(hipMemsetAsync(q->in[1] - interior_offset(q->n2, q->nv), 0, q->bytes / 2,
+                             run_stream(S, q)));
     HIP_TRY(hipStreamWaitEvent(run_stream(S, q), S->up_done[u], 0));  // steps after the upload
     if(q->cfl)                           // the first step's rates: those of the uploaded state
       HIP_TRY(launch_cfl(q->module, q->in[0], q->s, q->P, S->diags + q->diag_slot, run_stream(S, q)));
@@ -1958,8 +1963,10 @@ FusedShape server_shape(const fg2_server *S, int m)
   return best;
 }
 
-// Per-run mode: every active run's own one-pass step, launched on its pool
-// stream with the shape a single run of its size takes (fg2_step's).
+// Per-run modes: every active run's own step on its pool stream, either the
+// one-pass step with the shape a single run of its size takes (fg2_step's)
+// or the plain path (fg2_step's FG2_PATH_PLAIN: u in place, v in the run's
+// second array).
 int server_launch_per_run(fg2_server *S)
 {
   const int slot = S->tick_slot;
@@ -1968,9 +1975,20 @@ int server_launch_per_run(fg2_server *S)
     HIP_TRY(hipEventSynchronize(S->tick_ev[slot][k]));     // at most SLOTS ticks queued
   for(Request *q : S->active) {
     if(q->done_steps >= q->steps) continue;
-    HIP_TRY(launch_step(q->module, q->in[q->cur], q->in[q->cur ^ 1], q->s, q->P, q->dt, q->shape,
-                        0, S->diags + q->diag_slot, S->pool[q->sidx]));
-    q->cur ^= 1;
+    hipStream_t st = S->pool[q->sidx];
+    DeviceDiag *diag = S->diags + q->diag_slot;
+    if(S->mode == FG2_SERVE_PER_RUN_PLAIN) {
+      float *u = q->in[q->cur], *v = q->in[q->cur ^ 1];
+      for(int k = 0; k < 3; ++k) {
+        HIP_TRY(launch_bcond(q->module, u, q->s, q->P, q->P.p1, q->P.p2, st));
+        HIP_TRY(launch_kick(q->module, v, u, q->s, q->P, (float)RK_BETA[k], st));
+        HIP_TRY(launch_drift(q->module, u, v, q->s, q->P, (float)(q->dt * RK_GAMMA[k]), diag, st));
+      }
+    } else {
+      HIP_TRY(launch_step(q->module, q->in[q->cur], q->in[q->cur ^ 1], q->s, q->P, q->dt, q->shape,
+                          0, diag, st));
+      q->cur ^= 1;
+    }
     ++q->done_steps;
     S->cell_steps += (long long)q->n1 * q->n2;
   }
@@ -1980,7 +1998,7 @@ int server_launch_per_run(fg2_server *S)
 
 int server_launch(fg2_server *S)
 {
-  if(S->mode == FG2_SERVE_PER_RUN) return server_launch_per_run(S);
+  if(S->mode != FG2_SERVE_BATCHED) return server_launch_per_run(S);
   for(int m = 0; m < module_count(); ++m) {        // one launch per module with active runs
     hipStream_t ms = S->mstream[m];
     S->rec_host.clear();
@@ -2204,7 +2222,7 @@ int fg2_server_submit_run(fg2_server *S, const fg2_run *like, int steps, const f
   const size_t need = 2 * (size_t)array_floats(n1, n2, nvar_of(m)) * sizeof(float);
   if(need > S->budget) return fail(FG2_ERR_NOMEM, "request needs %zu bytes, budget %zu", need, S->budget);
   const bool cfl = like->dt_mode == FG2_DT_CFL, fo = first_order_active(like);
-  if(S->mode == FG2_SERVE_PER_RUN && (cfl || fo))
+  if(S->mode != FG2_SERVE_BATCHED && (cfl || fo))
     return fail(FG2_ERR_ARG, "per-run mode serves fixed-dt runs without a first_order pre-step");
   Request *q = new(std::nothrow) Request;
   if(!q) return fail(FG2_ERR_NOMEM, "host allocation failed");
@@ -2261,7 +2279,9 @@ int fg2_server_result(fg2_server *S, long long id, float *out, double *times, in
 
 int fg2_server_set_mode(fg2_server *S, int mode)
 {
-  if(!S || (mode != FG2_SERVE_BATCHED && mode != FG2_SERVE_PER_RUN)) return fail(FG2_ERR_ARG, "bad server mode");
+  if(!S || (mode != FG2_SERVE_BATCHED && mode != FG2_SERVE_PER_RUN &&
+            mode != FG2_SERVE_PER_RUN_PLAIN))
+    return fail(FG2_ERR_ARG, "bad server mode");
   if(!S->active.empty() || !S->queue.empty() || !S->uploading.empty())
     return fail(FG2_ERR_ARG, "the mode changes only while the server is idle");
   S->mode = mode;

@@ -365,9 +365,13 @@ int fg2_server_stats(fg2_server *server, long long *cell_steps, size_t *bytes_in
 /* How a tick advances the active runs: FG2_SERVE_BATCHED (default), one
  * batched one-pass launch per module; FG2_SERVE_PER_RUN, each run's own
  * one-pass step launch (the shape fg2_step gives a run of its size) on one
- * of 8 streams -- the baseline the batched mode is measured against, with
- * the same admission, arena and copies.  Changes only while idle. */
-enum { FG2_SERVE_BATCHED = 0, FG2_SERVE_PER_RUN = 1 };
+ * of 8 streams; FG2_SERVE_PER_RUN_PLAIN, each run's plain kernels (ghost
+ * fill, kick, drift per substage, the reference's operation split, in place
+ * on the run's state with its second array as v) on one of 8 streams.  The
+ * per-run modes are the baselines the batched mode is measured against,
+ * with the same admission, arena and copies, and give the same bits.
+ * Changes only while idle. */
+enum { FG2_SERVE_BATCHED = 0, FG2_SERVE_PER_RUN = 1, FG2_SERVE_PER_RUN_PLAIN = 2 };
 int fg2_server_set_mode(fg2_server *server, int mode);
 
 #ifdef __cplusplus

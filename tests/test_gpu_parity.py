@@ -334,6 +334,45 @@ def test_server_alone_vs_mix(fg2, waves, tile_h):
         np.testing.assert_array_equal(y, r.download())
 
 
+@pytest.mark.parametrize("mode", ["per_run", "per_run_plain"])
+def test_server_modes_same_bits(fg2, mode):
+    """The per-run serving baselines (each run's own one-pass step kernel, or
+    its plain bcond / kick / drift kernels, on the server's stream pool)
+    give the same bits as the batched mode, under the same budget and
+    arrivals; the plain mode starts v at zero in an arena another run used."""
+    import time
+    code = {"per_run": fg2.SERVE_PER_RUN, "per_run_plain": fg2.SERVE_PER_RUN_PLAIN}[mode]
+    reqs = [("hydro", 64, 96, 5), ("hydro-isot", 128, 64, 8), ("hydro", 100, 70, 3),
+            ("hydro-isot", 70, 130, 6), ("hydro", 200, 180, 4), ("hydro-isot", 96, 96, 1)]
+    xs = [state(m, a, b, seed=70 + k) for k, (m, a, b, _) in enumerate(reqs)]
+    budget = 3 * 4 * 4 * 220 * 220 + (1 << 16)     # forces queueing and arena reuse
+    out = {}
+    for name, m_ in (("batched", fg2.SERVE_BATCHED), (mode, code)):
+        _say(f"case server mode {name}")
+        srv = fg2.Server(budget)
+        srv.set_mode(m_)
+        ids = []
+        t0 = time.perf_counter()
+        for k, ((m, a, b, n), x) in enumerate(zip(reqs, xs)):
+            ids.append(srv.submit(m, x, n, time.perf_counter() - t0))
+            srv.tick(time.perf_counter() - t0)
+        for _ in range(500):
+            act, que, fin = srv.tick(time.perf_counter() - t0)
+            if fin == len(reqs) and act == 0:
+                break
+        assert fin == len(reqs)
+        res = []
+        for i in ids:
+            st, times, y = srv.result(i)
+            assert st == 0, (name, i, st)
+            assert np.all(np.isfinite(y))
+            res.append(y)
+        out[name] = res
+        srv.close()
+    for k, (a, b) in enumerate(zip(out["batched"], out[mode])):
+        np.testing.assert_array_equal(a, b, err_msg=f"request {k} {reqs[k]}")
+
+
 def test_disk_precision_small(fg2):
     """Precision study at reduced size (the full 2048^2 x 2000-step study is
     tests/precision_study.py): FP32 GPU vs FP64 oracle, disk 192^2, 200

@@ -3,7 +3,8 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--no-batch] [--no-serve] [--no-disk] [--no-spectrum] [--no-slab]
                     [--no-bounded] [--no-config1] [--no-sweep] [--disk-only]
-                    [--slab-only] [--bounded-only] [--headline-only]
+                    [--slab-only] [--bounded-only] [--headline-only] [--batch-only]
+                    [--serve-only]
 
 Workload (the `value`): hydro module, Kelvin-Helmholtz shear layer on a
 3584 x 3584 periodic grid, FP32, one launch of the one-pass step kernel per
@@ -666,6 +667,10 @@ def main():
                     help="run only the bounded-grid leg and print its JSON (profiling)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline step and its JSON line (profiling)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched leg and print its JSON (profiling)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving leg and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -683,6 +688,12 @@ def main():
     if args.bounded_only:
         print(json.dumps({"bounded": bench_bounded(ev, steps=args.steps, warmup=args.warmup)}),
               flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batched": bench_batched(ev)}), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps({"serving": bench_serving()}), flush=True)
         return
     x = ics.hydro_kh(N, N, seed=2)
     run = fg2_amd.Run("hydro", N, N, geometry=dict(l1=1.0, l2=1.0, p1=1, p2=1, c=0.5))

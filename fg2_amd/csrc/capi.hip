@@ -146,6 +146,7 @@ struct fg2_run {
   int path = FG2_PATH_FUSED, dt_mode = FG2_DT_CLOSED;
   FusedShape shape{0, 1};
   bool tile_auto = true;
+  int kernel_pick = 0;           // one-pass modules: 0 by rule, 1 one-pass, 2 substage (autotuned)
   uint64_t rng = 0;
   float rand1 = 0, rand2 = 0;
   bool lam_valid = false;        // diag holds the CFL rates of the current state
@@ -200,12 +201,14 @@ int batch_grid(int module, bool step, int waves, int cus, bool cfl = false)
 }
 
 // Substage kernels: a tile is one wavefront's 58-column strip, loading 6
-// halo rows besides its own.
+// halo rows besides its own.  A grid that fits one round with one- or
+// two-row tiles takes them (hydro 128^2 .. 512^2: 10-18 us per step against
+// 24-27 us with 8-row tiles).
 int auto_tile_h(const fg2_run *r, int waves)
 {
   const long nstrips = (r->n2 + STRIP - 1) / STRIP;
   const long slots = (long)blocks_per_cu(r->module, 0, waves) * r->cus * waves;
-  return tile_rows(r->n1, nstrips, slots, 8, 3);
+  return tile_rows(r->n1, nstrips, slots, 1, 3);
 }
 
 Params make_params(const fg2_run *r)
@@ -278,10 +281,28 @@ FusedShape shape_of(const fg2_run *r)
   return sh;
 }
 
-// The one-pass step kernel serves the Cartesian modules, periodic or not.
+FusedShape step_shape_of(const fg2_run *r);
+
+// A grid that fills under half of one round of the one-pass kernel's
+// workgroups is latency bound: a one-pass tile takes tile_h + 12 dependent
+// iterations of three stages, while the substage path's one- or two-row
+// tiles take three short launches.  Hydro 128^2, 256^2, 512^2 step in 10.2,
+// 12.0 and 18.1 us on the substage path against 12.9, 15.9 and 21.1 us on
+// the one-pass kernel's small-grid shape.
+bool latency_bound(const fg2_run *r)
+{
+  if(!r->tile_auto || r->slab_n1g) return false;
+  return step_shape_of(r).waves == 1;
+}
+
+// The one-pass step kernel serves the Cartesian modules, periodic or not,
+// except latency-bound grids with the automatic shape (the substage path)
+// and runs whose autotuned shape is a substage one.
 bool uses_step_kernel(const fg2_run *r)
 {
-  return r->path == FG2_PATH_FUSED && module_info(r->module).one_pass;
+  if(r->path != FG2_PATH_FUSED || !module_info(r->module).one_pass) return false;
+  if(r->kernel_pick) return r->kernel_pick == 1 || r->slab_n1g;
+  return !latency_bound(r);
 }
 
 FusedShape step_shape_of(const fg2_run *r)
@@ -742,6 +763,7 @@ int fg2_set_tile(fg2_run *r, int tile_h, int waves)
   if(waves != 1 && waves != 2 && waves != 4) return fail(FG2_ERR_SHAPE, "waves must be 1, 2 or 4");
   if(tile_h < 0) return fail(FG2_ERR_SHAPE, "tile_h must be >= 0");
   r->tile_auto = tile_h == 0;
+  r->kernel_pick = 0;
   r->shape.tile_h = tile_h;
   r->shape.waves = waves;
   return FG2_OK;
@@ -770,7 +792,11 @@ struct TuneKey {
   bool operator<(const TuneKey &o) const
   { return std::tie(module, n1, n2, step) < std::tie(o.module, o.n1, o.n2, o.step); }
 };
-std::map<TuneKey, FusedShape> g_tune_cache;
+struct TunePick {
+  FusedShape sh{0, 0};
+  bool step = false;
+};
+std::map<TuneKey, TunePick> g_tune_cache;
 std::mutex g_tune_mutex;
 
 // Time of one step with shape `sh`: the one-pass kernel, or the three
@@ -818,35 +844,39 @@ extern "C" int fg2_autotune(fg2_run *r, int *tile_h, int *waves)
 {
   if(int e = check_run(r)) return e;
   if(r->path == FG2_PATH_PLAIN) return fail(FG2_ERR_ARG, "the plain path has no tile shape");
+  // a latency-bound grid of a one-pass module times both kernels
+  r->kernel_pick = 0;
+  const bool one_pass = module_info(r->module).one_pass;
+  const bool both = one_pass && latency_bound(r);
   const bool step = uses_step_kernel(r);
-  const TuneKey key{r->module, r->n1, r->n2, step ? (r->p1 && r->p2 ? 1 : 2) : 0};
-  FusedShape best{0, 0};
+  const TuneKey key{r->module, r->n1, r->n2, both ? 3 : step ? (r->p1 && r->p2 ? 1 : 2) : 0};
+  TunePick best;
   {
     std::lock_guard<std::mutex> lk(g_tune_mutex);
     auto it = g_tune_cache.find(key);
     if(it != g_tune_cache.end()) best = it->second;
   }
-  if(best.tile_h == 0) {
+  if(best.sh.tile_h == 0) {
     // the NaN flag of steps queued before the tuning is kept
     HIP_TRY(hipMemcpyAsync(r->host_diag, r->diag, sizeof(DeviceDiag), hipMemcpyDeviceToHost, r->stream));
     HIP_TRY(hipStreamSynchronize(r->stream));
     const unsigned int flags = r->host_diag->flags;
-    std::vector<FusedShape> cand;
-    const FusedShape a = step ? step_shape_of(r) : shape_of(r);
-    cand.push_back(a);
+    std::vector<TunePick> cand;
+    if(both) cand.push_back({step_shape_of(r), true});
+    cand.push_back({step ? step_shape_of(r) : shape_of(r), step});
     const int hs_step[] = {32, 48, 64, 96, 112, 128, 160, 224};
-    const int hs_sub[] = {16, 24, 32, 48, 64, 128};
+    const int hs_sub[] = {1, 2, 4, 8, 16, 24, 32, 48, 64, 128};
     if(step) {
       for(int w : {4, 2})
-        for(int h : hs_step) if(h <= r->n1) cand.push_back({h, w});
+        for(int h : hs_step) if(h <= r->n1) cand.push_back({{h, w}, true});
     } else {
       for(int w : {4, 2, 1})
-        for(int h : hs_sub) if(h <= r->n1) cand.push_back({h, w});
+        for(int h : hs_sub) if(h <= r->n1) cand.push_back({{h, w}, false});
     }
     float best_ms = 1e30f;
-    for(const FusedShape &c : cand) {
+    for(const TunePick &c : cand) {
       float ms;
-      if(int e = time_launch(r, c, step, &ms)) return e;
+      if(int e = time_launch(r, c.sh, c.step, &ms)) return e;
       if(ms < best_ms) { best_ms = ms; best = c; }
     }
     restore_diag_kernel<<<1, 1, 0, r->stream>>>(r->diag, flags);
@@ -854,11 +884,12 @@ extern "C" int fg2_autotune(fg2_run *r, int *tile_h, int *waves)
     std::lock_guard<std::mutex> lk(g_tune_mutex);
     g_tune_cache[key] = best;
   }
-  r->shape = best;
+  r->shape = best.sh;
   r->tile_auto = false;
+  if(one_pass) r->kernel_pick = best.step ? 1 : 2;
   r->lam_valid = false;
-  if(tile_h) *tile_h = best.tile_h;
-  if(waves) *waves = best.waves;
+  if(tile_h) *tile_h = best.sh.tile_h;
+  if(waves) *waves = best.sh.waves;
   return FG2_OK;
 }
 

@@ -182,13 +182,18 @@ int fg2_set_dt_mode(fg2_run *run, int mode);
 /* Fused-kernel tile: rows per tile and wavefronts per workgroup (1, 2 or
  * 4; the one-pass step kernel gives each workgroup 64 waves - 18 output
  * columns, and a last strip of at most 46 columns one wavefront per row
- * chunk; the substage kernel gives each wavefront 58).  tile_h = 0 restores
- * the automatic choice (or the autotuned one, see fg2_autotune). */
+ * chunk; the substage kernel gives each wavefront 58).  A tile set here
+ * steps a module that has the one-pass kernel with that kernel.  tile_h = 0
+ * restores the automatic choice, which steps a grid too small to fill half
+ * of the GPU with one-pass tiles (hydro up to about 512^2) with the
+ * substage kernels and one- or two-row tiles. */
 int fg2_set_tile(fg2_run *run, int tile_h, int waves);
 int fg2_get_tile(const fg2_run *run, int *tile_h, int *waves);
 /* Time the run's fused kernel on its own state for candidate tile shapes
  * and keep the fastest (cached per module and grid size for the process);
- * replaces the block-size heuristic of setup() (src/setup.cpp:39-58).  The
+ * replaces the block-size heuristic of setup() (src/setup.cpp:39-58).  For
+ * a grid that the automatic choice steps with the substage kernels, the
+ * one-pass kernel's shape is timed too and the faster kernel kept.  The
  * state is unchanged; *tile_h, *waves (may be NULL) receive the choice. */
 int fg2_autotune(fg2_run *run, int *tile_h, int *waves);
 

@@ -304,6 +304,31 @@ def test_autotune_keeps_state_and_bits(fg2, per):
     np.testing.assert_array_equal(a.download(), b.download())
 
 
+@pytest.mark.parametrize("module", ["hydro", "hydro-isot"])
+def test_small_grid_takes_short_substage_tiles(fg2, module):
+    """A grid too small to fill the GPU with one-pass tiles steps with the
+    substage kernels and one- or two-row tiles when the shape is automatic
+    (bits of the substage path); the autotuner times both kernels and keeps
+    the bits; a large grid keeps the one-pass kernel's four-wavefront tiles."""
+    n1, n2 = 128, 160
+    _say(f"case small grid {module} {n1}x{n2}")
+    x = state(module, n1, n2, seed=37)
+    ref = _run(fg2, module, n1, n2, x, fg2.PATH_SUBSTAGE)
+    dt = 0.25 * ref.getdt()
+    assert ref.run(0.0, dt, 3) == 0
+    a = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED)
+    th, w = a.tile()
+    assert th <= 2 and w == 1, (th, w)
+    assert a.run(0.0, dt, 3) == 0
+    np.testing.assert_array_equal(a.download(), ref.download())
+    b = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED)
+    _say("autotuned", b.autotune())
+    assert b.run(0.0, dt, 3) == 0
+    np.testing.assert_array_equal(b.download(), ref.download())
+    big = fg2.Run(module, 2048, 2048)
+    assert big.tile()[1] == 4
+
+
 @pytest.mark.parametrize("waves,tile_h", [(4, 48), (0, 0)])
 def test_server_alone_vs_mix(fg2, waves, tile_h):
     """Continuous batching: requests of both modules and different sizes and

@@ -135,6 +135,10 @@ def bench_batched(ev, steps=500, nruns=64, seed=7):
 
     runs = [fg2_amd.Run("hydro-isot", int(a), int(b)) for a, b in zip(n1s, n2s)]
     for r, x in zip(runs, xs):
+        # the one-pass kernel on every run: one launch per step.  The small
+        # runs' automatic substage kernels (three launches) are faster alone
+        # but slower side by side.
+        r.set_kernel(fg2_amd.KERNEL_ONE_PASS)
         r.upload(x)
     dts = [r.getdt() for r in runs]
     # per-run launches

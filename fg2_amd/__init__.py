@@ -77,6 +77,7 @@ FIELDS = _Modules(lambda nv, fields: fields)    # module name -> state variable 
 
 OK, ERR_NAN, ERR_HIP, ERR_SHAPE, ERR_ARG, ERR_NOMEM = range(6)
 PATH_FUSED, PATH_PLAIN, PATH_SUBSTAGE = 0, 1, 2
+KERNEL_AUTO, KERNEL_ONE_PASS, KERNEL_SUBSTAGE = 0, 1, 2
 DT_CLOSED, DT_CFL = 0, 1
 SERVE_BATCHED, SERVE_PER_RUN, SERVE_PER_RUN_PLAIN = 0, 1, 2
 
@@ -131,6 +132,7 @@ def _load():
         "fg2_set_dt_mode": (I, [P, I]),
         "fg2_set_tile": (I, [P, I, I]),
         "fg2_get_tile": (I, [P, IP, IP]),
+        "fg2_set_kernel": (I, [P, I]),
         "fg2_autotune": (I, [P, IP, IP]),
         "fg2_step": (I, [P, D, D]),
         "fg2_step_async": (I, [P, D, D]),
@@ -289,6 +291,11 @@ class Run:
 
     def set_tile(self, tile_h, waves=1):
         _check(lib.fg2_set_tile(self.h, int(tile_h), int(waves)))
+
+    def set_kernel(self, kernel):
+        """Fix the fused path's kernel (KERNEL_AUTO, KERNEL_ONE_PASS,
+        KERNEL_SUBSTAGE) of a hydro or hydro-isot run."""
+        _check(lib.fg2_set_kernel(self.h, int(kernel)))
 
     def autotune(self):
         a, b = ctypes.c_int(), ctypes.c_int()

@@ -146,7 +146,7 @@ struct fg2_run {
   int path = FG2_PATH_FUSED, dt_mode = FG2_DT_CLOSED;
   FusedShape shape{0, 1};
   bool tile_auto = true;
-  int kernel_pick = 0;           // one-pass modules: 0 by rule, 1 one-pass, 2 substage (autotuned)
+  int kernel_pick = FG2_KERNEL_AUTO;   // fused-path kernel of a one-pass module
   uint64_t rng = 0;
   float rand1 = 0, rand2 = 0;
   bool lam_valid = false;        // diag holds the CFL rates of the current state
@@ -296,12 +296,13 @@ bool latency_bound(const fg2_run *r)
 }
 
 // The one-pass step kernel serves the Cartesian modules, periodic or not,
-// except latency-bound grids with the automatic shape (the substage path)
-// and runs whose autotuned shape is a substage one.
+// except latency-bound grids with the automatic shape (the substage path),
+// unless the run's kernel is fixed (slabs always take the one-pass kernel).
 bool uses_step_kernel(const fg2_run *r)
 {
   if(r->path != FG2_PATH_FUSED || !module_info(r->module).one_pass) return false;
-  if(r->kernel_pick) return r->kernel_pick == 1 || r->slab_n1g;
+  if(r->slab_n1g || r->kernel_pick == FG2_KERNEL_ONE_PASS) return true;
+  if(r->kernel_pick == FG2_KERNEL_SUBSTAGE) return false;
   return !latency_bound(r);
 }
 
@@ -763,9 +764,19 @@ int fg2_set_tile(fg2_run *r, int tile_h, int waves)
   if(waves != 1 && waves != 2 && waves != 4) return fail(FG2_ERR_SHAPE, "waves must be 1, 2 or 4");
   if(tile_h < 0) return fail(FG2_ERR_SHAPE, "tile_h must be >= 0");
   r->tile_auto = tile_h == 0;
-  r->kernel_pick = 0;
   r->shape.tile_h = tile_h;
   r->shape.waves = waves;
+  return FG2_OK;
+}
+
+int fg2_set_kernel(fg2_run *r, int kernel)
+{
+  if(!r) return fail(FG2_ERR_ARG, "NULL run");
+  if(kernel != FG2_KERNEL_AUTO && kernel != FG2_KERNEL_ONE_PASS && kernel != FG2_KERNEL_SUBSTAGE)
+    return fail(FG2_ERR_ARG, "bad kernel choice %d", kernel);
+  if(kernel == FG2_KERNEL_ONE_PASS && !module_info(r->module).one_pass)
+    return fail(FG2_ERR_ARG, "this module has no one-pass step kernel");
+  r->kernel_pick = kernel;
   return FG2_OK;
 }
 
@@ -844,10 +855,10 @@ extern "C" int fg2_autotune(fg2_run *r, int *tile_h, int *waves)
 {
   if(int e = check_run(r)) return e;
   if(r->path == FG2_PATH_PLAIN) return fail(FG2_ERR_ARG, "the plain path has no tile shape");
-  // a latency-bound grid of a one-pass module times both kernels
-  r->kernel_pick = 0;
+  // a latency-bound grid of a one-pass module with the automatic kernel
+  // choice times both kernels
   const bool one_pass = module_info(r->module).one_pass;
-  const bool both = one_pass && latency_bound(r);
+  const bool both = one_pass && r->kernel_pick == FG2_KERNEL_AUTO && latency_bound(r);
   const bool step = uses_step_kernel(r);
   const TuneKey key{r->module, r->n1, r->n2, both ? 3 : step ? (r->p1 && r->p2 ? 1 : 2) : 0};
   TunePick best;
@@ -886,7 +897,7 @@ extern "C" int fg2_autotune(fg2_run *r, int *tile_h, int *waves)
   }
   r->shape = best.sh;
   r->tile_auto = false;
-  if(one_pass) r->kernel_pick = best.step ? 1 : 2;
+  if(one_pass) r->kernel_pick = best.step ? FG2_KERNEL_ONE_PASS : FG2_KERNEL_SUBSTAGE;
   r->lam_valid = false;
   if(tile_h) *tile_h = best.sh.tile_h;
   if(waves) *waves = best.sh.waves;

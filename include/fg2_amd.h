@@ -46,6 +46,15 @@ enum {
   FG2_PATH_SUBSTAGE = 2  /* one fused kick+drift launch per substage, always         */
 };
 
+/* Kernel of the fused path for hydro and hydro-isot (fg2_set_kernel). */
+enum {
+  FG2_KERNEL_AUTO     = 0, /* default: the one-pass step kernel, except that a grid
+                              too small to fill half of the GPU with one-pass tiles
+                              and an automatic tile takes the substage kernels     */
+  FG2_KERNEL_ONE_PASS = 1, /* the one-pass step kernel, whatever the grid size     */
+  FG2_KERNEL_SUBSTAGE = 2  /* one fused kick+drift launch per substage             */
+};
+
 /* Time-step rule used by fg2_solve. */
 enum {
   FG2_DT_CLOSED = 0,  /* c / (n1/l1 + n2/l2), src/getdt.cpp:26 (default)               */
@@ -182,19 +191,26 @@ int fg2_set_dt_mode(fg2_run *run, int mode);
 /* Fused-kernel tile: rows per tile and wavefronts per workgroup (1, 2 or
  * 4; the one-pass step kernel gives each workgroup 64 waves - 18 output
  * columns, and a last strip of at most 46 columns one wavefront per row
- * chunk; the substage kernel gives each wavefront 58).  A tile set here
- * steps a module that has the one-pass kernel with that kernel.  tile_h = 0
- * restores the automatic choice, which steps a grid too small to fill half
- * of the GPU with one-pass tiles (hydro up to about 512^2) with the
- * substage kernels and one- or two-row tiles. */
+ * chunk; the substage kernel gives each wavefront 58).  With the automatic
+ * kernel choice, a tile set here steps a module that has the one-pass
+ * kernel with that kernel; tile_h = 0 restores the automatic shape, which
+ * steps a grid too small to fill half of the GPU with one-pass tiles (hydro
+ * up to about 512^2) with the substage kernels and one- or two-row tiles. */
 int fg2_set_tile(fg2_run *run, int tile_h, int waves);
 int fg2_get_tile(const fg2_run *run, int *tile_h, int *waves);
+/* Fix the fused path's kernel of a hydro or hydro-isot run (FG2_KERNEL_*;
+ * FG2_KERNEL_ONE_PASS on disk is FG2_ERR_ARG).  New: the reference's step
+ * has one form (src/step.cpp:22-43).  A single small run steps faster on
+ * the substage kernels; many small runs launched side by side on their own
+ * streams go faster on the one-pass kernel, one launch per step. */
+int fg2_set_kernel(fg2_run *run, int kernel);
 /* Time the run's fused kernel on its own state for candidate tile shapes
  * and keep the fastest (cached per module and grid size for the process);
- * replaces the block-size heuristic of setup() (src/setup.cpp:39-58).  For
- * a grid that the automatic choice steps with the substage kernels, the
- * one-pass kernel's shape is timed too and the faster kernel kept.  The
- * state is unchanged; *tile_h, *waves (may be NULL) receive the choice. */
+ * replaces the block-size heuristic of setup() (src/setup.cpp:39-58).  With
+ * the automatic kernel choice, a grid that it steps with the substage
+ * kernels has the one-pass kernel's shape timed too, and the faster kernel
+ * becomes the run's kernel (fg2_set_kernel).  The state is unchanged;
+ * *tile_h, *waves (may be NULL) receive the choice. */
 int fg2_autotune(fg2_run *run, int *tile_h, int *waves);
 
 /* One low-storage RK3 step of size dt at time t: first_order, then three

@@ -309,7 +309,9 @@ def test_small_grid_takes_short_substage_tiles(fg2, module):
     """A grid too small to fill the GPU with one-pass tiles steps with the
     substage kernels and one- or two-row tiles when the shape is automatic
     (bits of the substage path); the autotuner times both kernels and keeps
-    the bits; a large grid keeps the one-pass kernel's four-wavefront tiles."""
+    the bits; a fixed one-pass kernel keeps them too; a large grid keeps the
+    one-pass kernel's four-wavefront tiles unless the substage kernel is
+    fixed; disk has no one-pass kernel to fix."""
     n1, n2 = 128, 160
     _say(f"case small grid {module} {n1}x{n2}")
     x = state(module, n1, n2, seed=37)
@@ -325,8 +327,18 @@ def test_small_grid_takes_short_substage_tiles(fg2, module):
     _say("autotuned", b.autotune())
     assert b.run(0.0, dt, 3) == 0
     np.testing.assert_array_equal(b.download(), ref.download())
+    c = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED)
+    c.set_kernel(fg2.KERNEL_ONE_PASS)         # the one-pass small-grid shape
+    assert c.tile()[1] == 1
+    assert c.run(0.0, dt, 3) == 0
+    np.testing.assert_array_equal(c.download(), ref.download())
     big = fg2.Run(module, 2048, 2048)
     assert big.tile()[1] == 4
+    big.set_kernel(fg2.KERNEL_SUBSTAGE)
+    assert big.tile()[1] == 1
+    d = fg2.Run("disk", 64, 64, params=params_for("disk"))
+    with pytest.raises(fg2.Fg2Error):
+        d.set_kernel(fg2.KERNEL_ONE_PASS)
 
 
 @pytest.mark.parametrize("waves,tile_h", [(4, 48), (0, 0)])

@@ -306,7 +306,8 @@ def bench_serving(nreq=1000, rate=100.0, seed=11, budget_gib=8.0, waves=0, tile_
                        "speedup_vs_per_run": cap["batched"]["cell_updates_per_s"] /
                                              cap["per_run"]["cell_updates_per_s"],
                        "per_run_note": "each admitted run launches its own one-pass step "
-                                       "kernel (the shape fg2_step gives a run of its "
+                                       "kernel (its automatic one-pass shape, as "
+                                       "fg2_set_kernel(FG2_KERNEL_ONE_PASS) gives a run of its "
                                        "size) per tick, on 8 streams; same admission, "
                                        "arena and copies",
                        "plain": {"workload": f"the first {nplain} requests of the stream, all "

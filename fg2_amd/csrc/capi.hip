@@ -2006,9 +2006,9 @@ FusedShape server_shape(const fg2_server *S, int m)
 }
 
 // Per-run modes: every active run's own step on its pool stream, either the
-// one-pass step with the shape a single run of its size takes (fg2_step's)
-// or the plain path (fg2_step's FG2_PATH_PLAIN: u in place, v in the run's
-// second array).
+// one-pass step with the one-pass kernel's automatic shape for a single run
+// of its size (fg2_step's with FG2_KERNEL_ONE_PASS) or the plain path
+// (fg2_step's FG2_PATH_PLAIN: u in place, v in the run's second array).
 int server_launch_per_run(fg2_server *S)
 {
   const int slot = S->tick_slot;

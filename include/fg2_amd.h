@@ -385,8 +385,9 @@ int fg2_server_stats(fg2_server *server, long long *cell_steps, size_t *bytes_in
 
 /* How a tick advances the active runs: FG2_SERVE_BATCHED (default), one
  * batched one-pass launch per module; FG2_SERVE_PER_RUN, each run's own
- * one-pass step launch (the shape fg2_step gives a run of its size) on one
- * of 8 streams; FG2_SERVE_PER_RUN_PLAIN, each run's plain kernels (ghost
+ * one-pass step launch (the one-pass kernel's automatic shape for a run of
+ * its size, as with FG2_KERNEL_ONE_PASS) on one of 8 streams;
+ * FG2_SERVE_PER_RUN_PLAIN, each run's plain kernels (ghost
  * fill, kick, drift per substage, the reference's operation split, in place
  * on the run's state with its second array as v) on one of 8 streams.  The
  * per-run modes are the baselines the batched mode is measured against,

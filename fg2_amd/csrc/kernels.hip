@@ -389,11 +389,13 @@ void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, fl
 //   stage 2 on row r0 - 6              (input b = stage-1 output, ring W2),
 // each stage exactly the per-cell arithmetic of the substage kernel.  The
 // stage v's travel in small register rings (V0, V1); only u is read and the
-// new u written, so HBM traffic per step is 2 NVAR floats per cell.  Rows and
+// new u written, so HBM traffic per step is 2 NVAR floats per cell.  (The
+// one-barrier schedule, below, lags stages 1 and 2 by four rows each.)  Rows and
 // columns outside the grid wrap (periodic), and the right-hand side sees the
 // wrapped indices, so every halo value equals the value the substage path
 // computes at the wrapped cell, bit for bit.  Horizontal neighbours of a
-// stage's centre row come from LDS (one s_barrier per stage).  A last strip
+// stage's centre row come from LDS: one s_barrier per stage, or one per
+// iteration for all three stages (step_one_barrier).  A last strip
 // of at most 46 columns is packed: each wavefront of its workgroups marches
 // its own row chunk on its own 64 lanes (step_tile_shape).
 //
@@ -451,33 +453,50 @@ __device__ __forceinline__ void ghost_rows(float (&W)[8][NV], const int S, const
 // instructions, 2.6% on the 3584^2 hydro step.  For NV = 3 they stay
 // variable-major: the wider reads would keep more registers live and cost
 // hydro-isot its third wavefront per SIMD.
-template<class M, int NW> struct StepLds {
+// NB = 2 buffers for the one-barrier schedule (iteration it writes buffer
+// it & 1 while slower wavefronts may still read the other).
+template<class M, int NW, int NB = 1> struct StepLds {
   static constexpr bool LANE_MAJOR = M::NV == 4 && M::ND == 2;
-  struct Lane { float4 X[3][64 * NW]; float2 G[3][3][64 * NW]; };
-  struct Var { float X[3][M::NV][64 * NW]; float G[3][3][M::ND][64 * NW]; };
+  struct Lane { float4 X[NB][3][64 * NW]; float2 G[NB][3][3][64 * NW]; };
+  struct Var { float X[NB][3][M::NV][64 * NW]; float G[NB][3][3][M::ND][64 * NW]; };
   std::conditional_t<LANE_MAJOR, Lane, Var> r;
 };
+
+// Schedule of the one-pass step of module M.  One barrier per row
+// iteration: stage 1 lags stage 0 and stage 2 lags stage 1 by four rows, so
+// each stage reads only rows produced in earlier iterations, and all three
+// publish their LDS rows (double-buffered) before one barrier; their LDS
+// reads then overlap each other's arithmetic.  It needs more registers
+// (longer-lived rows) and twice the LDS: the hydro step (two wavefronts per
+// SIMD either way) gains 8% (214.8 against 233.8 us per 3584^2 step);
+// hydro-isot would drop from three wavefronts per SIMD to two and lose 17%,
+// and the batched kernels, whose run records take registers of their own,
+// would lose occupancy, so they keep three barriers (one per stage).
+template<class M> constexpr bool step_one_barrier() { return M::STEP_ONE_BARRIER; }
 
 // One tile of the one-pass step: output rows [i0, i1), output columns
 // [j0, j0 + WL - 18) on the WL = 64 NW lanes of the workgroup, or (a packed
 // narrow strip: lane0 = 64 w, WL = 64) on wavefront w's own 64 lanes.  All
-// threads of the workgroup call it together with the same nit >= i1 - i0 + 12
-// (iterations past a wavefront's own rows store nothing).
+// threads of the workgroup call it together with the same nit0 >= i1 - i0 +
+// 12 (iterations past a wavefront's own rows store nothing; the one-barrier
+// schedule, ONE, runs two more).
 // STEP_SLAB: the run is rows [H.row0, H.row0 + n1) of a grid of H.n1g rows;
 // rows above and below it come from the halo buffers H.lo and H.hi instead
 // of wrapping, and the right-hand side sees the global row index.
 // STEP_BOUNDED: P.p1 / P.p2 select wrapped or mirrored rows / columns.
-template<class M, int NW, int MODE = STEP_PERIODIC>
+template<class M, int NW, int MODE = STEP_PERIODIC, bool ONE = false>
 __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *__restrict__ uout,
                                           const long s, const Params &P, const float g0,
                                           const float g1, const float g2, const float b1,
                                           const float b2, const int j0, const int i0,
                                           const int i1, const int lane0, const int WL,
-                                          const int nit, const int cfl, StepLds<M, NW> &L,
-                                          TileOut &res, const SlabHalo &H = SlabHalo{})
+                                          const int nit0, const int cfl,
+                                          StepLds<M, NW, ONE ? 2 : 1> &L, TileOut &res,
+                                          const SlabHalo &H = SlabHalo{})
 {
   constexpr int NV = M::NV, ND = M::ND, H3 = 3 * HALF;
   constexpr bool LM = StepLds<M, NW>::LANE_MAJOR;
+  const int nit = nit0 + (ONE ? 2 : 0);
   auto &X = L.r.X;
   auto &G = L.r.G;
   constexpr bool SLAB = MODE == STEP_SLAB, BOUNDED = MODE == STEP_BOUNDED;
@@ -516,18 +535,22 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
   // lanes tl <= n2 - j0 + 17 (its last output column plus the 9-column halo).
   // Wavefronts wholly past that, and wavefronts of a packed narrow strip
   // without rows, skip the work and keep only the workgroup's barriers,
-  // three per iteration as below.  The lanes next to them read stale LDS,
+  // three (or one) per iteration as below.  The lanes next to them read stale LDS,
   // which only reaches lanes more than 9 columns past the last output column.
   if(64 * __builtin_amdgcn_readfirstlane(t >> 6) - lane0 > n2 - j0 + 2 * H3 - 1 || i1 <= i0) {
-    for(int it = 0; it < nit; ++it) { __syncthreads(); __syncthreads(); __syncthreads(); }
+    for(int it = 0; it < nit; ++it) {
+      __syncthreads();
+      if constexpr(!ONE) { __syncthreads(); __syncthreads(); }
+    }
     return;
   }
 
   float W0[8][NV], W1[8][NV], W2[8][NV], V0[4][NV], V1[4][NV];
   auto load_row = [&](int r, float (&dst)[NV]) {
     if constexpr(SLAB) {
-      // r >= i0 - 9 >= -9 always; the last prefetch asks for row i1 + 9,
-      // which no stage reads, so it is clamped into the buffer
+      // r >= i0 - 9 >= -9 always; the last prefetches ask for rows up to
+      // i1 + 9 (i1 + 11 with one barrier), which no stage reads, so they
+      // are clamped into the buffer
       const float *base = r < 0 ? H.lo : r >= n1 ? H.hi : uin;
       const int rr = r < 0 ? r + SLAB_HALO : r >= n1 ? min(r - n1, SLAB_HALO - 1) : r;
       gload_cell<NV>(gptr(base + (long)rr * s + (long)cc * NV), dst);
@@ -540,32 +563,31 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
 #pragma unroll
   for(int k = 0; k < 7; ++k) load_row(i0 - H3 + k, W0[k]);
 
-  // one stage on a centre row whose window rows are win(-3..3)
-  auto stage = [&](const int k, auto &&win, const int i, const float (&vprev)[NV],
-                   const float beta, const float gdt, float (&vn)[NV], float (&uo)[NV]) {
+  // One stage on a centre row whose window rows are win(-3..3), in buffer
+  // bf: publish writes the centre row and its g_k to LDS; after a barrier,
+  // compute reads the neighbours and evaluates the right-hand side.
+  auto publish = [&](const int bf, const int k, auto &&win) {
     const float (&cen)[NV] = win(0);
-    // Issue priority: wavefronts past the barrier, with the stage's
-    // right-hand side ahead of them, go before wavefronts still publishing
-    // their row (measured 0.6-0.9% on the 3584^2 hydro step; the reverse
-    // order costs 2%)
-    __builtin_amdgcn_s_setprio(0);
     if constexpr(LM) {
-      X[k][t] = make_float4(cen[0], cen[1], cen[2], cen[3]);
+      X[bf][k][t] = make_float4(cen[0], cen[1], cen[2], cen[3]);
 #pragma unroll
       for(int kk = 1; kk <= 3; ++kk)
-        G[k][kk - 1][t] = make_float2(win(kk)[M::D12V[0]] - win(-kk)[M::D12V[0]],
-                                      win(kk)[M::D12V[1]] - win(-kk)[M::D12V[1]]);
+        G[bf][k][kk - 1][t] = make_float2(win(kk)[M::D12V[0]] - win(-kk)[M::D12V[0]],
+                                          win(kk)[M::D12V[1]] - win(-kk)[M::D12V[1]]);
     } else {
 #pragma unroll
-      for(int q = 0; q < NV; ++q) X[k][q][t] = cen[q];
+      for(int q = 0; q < NV; ++q) X[bf][k][q][t] = cen[q];
 #pragma unroll
       for(int kk = 1; kk <= 3; ++kk)
 #pragma unroll
         for(int d = 0; d < ND; ++d)
-          G[k][kk - 1][d][t] = win(kk)[M::D12V[d]] - win(-kk)[M::D12V[d]];
+          G[bf][k][kk - 1][d][t] = win(kk)[M::D12V[d]] - win(-kk)[M::D12V[d]];
     }
-    __syncthreads();
-    __builtin_amdgcn_s_setprio(1);
+  };
+  auto compute = [&](const int bf, const int k, auto &&win, const int i,
+                     const float (&vprev)[NV], const float beta, const float gdt,
+                     float (&vn)[NV], float (&uo)[NV]) {
+    const float (&cen)[NV] = win(0);
     typename M::P_t x;
 #pragma unroll
     for(int q = 0; q < NV; ++q) x.c[q] = cen[q];
@@ -575,18 +597,18 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
       // variable, or per lane from lane-major rows)
       auto xr = [&](int side, int q) -> float {
         if constexpr(LM) {
-          const float4 v = X[k][nb[side][kk - 1]];
+          const float4 v = X[bf][k][nb[side][kk - 1]];
           return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
         } else {
-          return X[k][q][nb[side][kk - 1]];
+          return X[bf][k][q][nb[side][kk - 1]];
         }
       };
       auto gr = [&](int side, int d) -> float {
         if constexpr(LM) {
-          const float2 v = G[k][kk - 1][nb[side][kk - 1]];
+          const float2 v = G[bf][k][kk - 1][nb[side][kk - 1]];
           return d == 0 ? v.x : v.y;
         } else {
-          return G[k][kk - 1][d][nb[side][kk - 1]];
+          return G[bf][k][kk - 1][d][nb[side][kk - 1]];
         }
       };
 #pragma unroll
@@ -614,7 +636,26 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
       uo[q] = __builtin_fmaf(gdt, vn[q], cen[q]);
     }
   };
+  // stage 2's output row r2: the new state
+  auto store = [&](const int r2, const float (&uo)[NV]) {
+    if(out && r2 < i1) {
+#pragma unroll
+      for(int q = 0; q < NV; ++q) res.bad |= (uo[q] != uo[q]);
+      gstore_cell<NV>(gptr(uout + (long)r2 * s + (long)c * NV), uo);
+      if(cfl) {
+        res.lam_adv = fmaxf(res.lam_adv, M::lam_adv(P, uo, r2, c, k1a, k1b));
+        if constexpr(M::CELL_DIFF) res.lam_diff = fmaxf(res.lam_diff, M::lam_diff(P, uo, r2, c));
+      }
+    }
+  };
 
+  // Row r of stage 0's output (W1) lives in slot (r - i0 + 6) & 7, of stage
+  // 1's (W2) in slot (r - i0 + 11) & 7 (three barriers) or (r - i0 + 12) & 7
+  // (one barrier); the v rings take one row per iteration, 4 slots.
+  // Issue priority: wavefronts past a barrier, with right-hand sides ahead
+  // of them, go before wavefronts still publishing their rows (measured
+  // 0.6-0.9% on the 3584^2 hydro step with three barriers; the reverse order
+  // costs 2%).
   for(int base = 0; base < nit; base += 8) {
 #pragma unroll
     for(int p = 0; p < 8; ++p) {
@@ -622,43 +663,82 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
       if(it >= nit) break;
       const int r0 = i0 - 2 * HALF + it;
       load_row(r0 + 4, W0[(p + 7) & 7]);
-      {                                          // stage 0 on row r0
+      auto w0 = [&](int d) -> float (&)[NV] { return W0[(p + 3 + d) & 7]; };
+      // stage 0 on row r0, its output into W1 and V0
+      auto stage0 = [&](const int bf) {
         float vn[NV], uo[NV];
-        stage(0, [&](int d) -> float (&)[NV] { return W0[(p + 3 + d) & 7]; }, r0, V0[0],
-              0.0f, g0, vn, uo);
+        compute(bf, 0, w0, r0, V0[0], 0.0f, g0, vn, uo);
 #pragma unroll
         for(int q = 0; q < NV; ++q) { W1[p & 7][q] = uo[q]; V0[p & 3][q] = vn[q]; }
         // Neumann rows: the output of row r0 lives in slot p & 7, that of
         // row r0 - m in slot (p - m) & 7
         if constexpr(BOUNDED) if(mirror_rows) ghost_rows(W1, p, r0, n1);
-      }
-      if(it >= 2 * HALF) {                       // stage 1 on row r0 - 3
-        float vn[NV], uo[NV];
-        stage(1, [&](int d) -> float (&)[NV] { return W1[(p + 5 + d) & 7]; }, r0 - HALF,
-              V0[(p + 1) & 3], b1, g1, vn, uo);
-#pragma unroll
-        for(int q = 0; q < NV; ++q) { W2[(p + 2) & 7][q] = uo[q]; V1[(p + 2) & 3][q] = vn[q]; }
-        // row r0 - 3 lives in slot (p + 2) & 7
-        if constexpr(BOUNDED) if(mirror_rows) ghost_rows(W2, p + 2, r0 - HALF, n1);
-      } else {
+      };
+      if constexpr(ONE) {
+        // stage 1 on row r0 - 4, stage 2 on row r0 - 8: each reads rows the
+        // stage before it produced in earlier iterations, so all three
+        // publish into buffer p & 1 before the one barrier.  A wavefront
+        // writes that buffer again two iterations on, after a barrier that
+        // every wavefront passes only once its reads of it are done.  Each
+        // stage reads its v slot before the stage before it refills the slot
+        // (stage 2 first).
+        const int bf = p & 1;
+        auto w1 = [&](int d) -> float (&)[NV] { return W1[(p + 4 + d) & 7]; };
+        auto w2 = [&](int d) -> float (&)[NV] { return W2[(p + 6 + d) & 7]; };
+        const bool s1 = it >= 2 * HALF + 1, s2 = it >= 4 * HALF + 2;
+        __builtin_amdgcn_s_setprio(0);
+        publish(bf, 0, w0);
+        if(s1) publish(bf, 1, w1);
+        if(s2) publish(bf, 2, w2);
         __syncthreads();
-      }
-      if(it >= 4 * HALF) {                       // stage 2 on row r0 - 6
-        float vn[NV], uo[NV];
-        const int r2 = r0 - 2 * HALF;
-        stage(2, [&](int d) -> float (&)[NV] { return W2[(p + 7 + d) & 7]; }, r2,
-              V1[(p + 3) & 3], b2, g2, vn, uo);
-        if(out && r2 < i1) {
-#pragma unroll
-          for(int q = 0; q < NV; ++q) res.bad |= (uo[q] != uo[q]);
-          gstore_cell<NV>(gptr(uout + (long)r2 * s + (long)c * NV), uo);
-          if(cfl) {
-            res.lam_adv = fmaxf(res.lam_adv, M::lam_adv(P, uo, r2, c, k1a, k1b));
-            if constexpr(M::CELL_DIFF) res.lam_diff = fmaxf(res.lam_diff, M::lam_diff(P, uo, r2, c));
-          }
+        __builtin_amdgcn_s_setprio(1);
+        if(s2) {
+          float vn[NV], uo[NV];
+          const int r2 = r0 - 2 * HALF - 2;
+          compute(bf, 2, w2, r2, V1[(p + 2) & 3], b2, g2, vn, uo);
+          store(r2, uo);
         }
+        if(s1) {
+          float vn[NV], uo[NV];
+          compute(bf, 1, w1, r0 - HALF - 1, V0[p & 3], b1, g1, vn, uo);
+#pragma unroll
+          for(int q = 0; q < NV; ++q) { W2[(p + 2) & 7][q] = uo[q]; V1[(p + 2) & 3][q] = vn[q]; }
+          // row r0 - 4 lives in slot (p + 2) & 7
+          if constexpr(BOUNDED) if(mirror_rows) ghost_rows(W2, p + 2, r0 - HALF - 1, n1);
+        }
+        stage0(bf);
       } else {
+        // stage 1 on row r0 - 3 and stage 2 on row r0 - 6 read the rows the
+        // stage before them has just produced: a barrier per stage
+        auto w1 = [&](int d) -> float (&)[NV] { return W1[(p + 5 + d) & 7]; };
+        auto w2 = [&](int d) -> float (&)[NV] { return W2[(p + 7 + d) & 7]; };
+        __builtin_amdgcn_s_setprio(0);
+        publish(0, 0, w0);
         __syncthreads();
+        __builtin_amdgcn_s_setprio(1);
+        stage0(0);
+        __builtin_amdgcn_s_setprio(0);
+        if(it >= 2 * HALF) publish(0, 1, w1);
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(1);
+        if(it >= 2 * HALF) {
+          float vn[NV], uo[NV];
+          compute(0, 1, w1, r0 - HALF, V0[(p + 1) & 3], b1, g1, vn, uo);
+#pragma unroll
+          for(int q = 0; q < NV; ++q) { W2[(p + 2) & 7][q] = uo[q]; V1[(p + 2) & 3][q] = vn[q]; }
+          // row r0 - 3 lives in slot (p + 2) & 7
+          if constexpr(BOUNDED) if(mirror_rows) ghost_rows(W2, p + 2, r0 - HALF, n1);
+        }
+        __builtin_amdgcn_s_setprio(0);
+        if(it >= 4 * HALF) publish(0, 2, w2);
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(1);
+        if(it >= 4 * HALF) {
+          float vn[NV], uo[NV];
+          const int r2 = r0 - 2 * HALF;
+          compute(0, 2, w2, r2, V1[(p + 3) & 3], b2, g2, vn, uo);
+          store(r2, uo);
+        }
       }
     }
   }
@@ -704,12 +784,13 @@ void step_kernel(const float *__restrict__ uin, float *__restrict__ uout, const 
                  const float b1, const float b2, const int tile_h, const int nfull,
                  const int cfl, DeviceDiag *diag)
 {
-  __shared__ StepLds<M, NW> L;
+  constexpr bool ONE = step_one_barrier<M>();
+  __shared__ StepLds<M, NW, ONE ? 2 : 1> L;
   const StepTileShape T = step_tile_shape<NW>(xcd_remap(blockIdx.x, gridDim.x), P.n1, tile_h,
                                               nfull);
   TileOut res{0, 0.0f, 0.0f};
-  step_tile<M, NW>(uin, uout, s, P, g0, g1, g2, b1, b2, T.j0, T.i0, T.i1, T.lane0, T.wl, T.nit,
-                   cfl, L, res);
+  step_tile<M, NW, STEP_PERIODIC, ONE>(uin, uout, s, P, g0, g1, g2, b1, b2, T.j0, T.i0, T.i1,
+                                       T.lane0, T.wl, T.nit, cfl, L, res);
   commit_tile(res, cfl, diag);
 }
 
@@ -723,12 +804,13 @@ void slab_step_kernel(const float *__restrict__ uin, float *__restrict__ uout, c
                       const float b1, const float b2, const int tile_h, const int nfull,
                       const int cfl, DeviceDiag *diag, const SlabHalo H)
 {
-  __shared__ StepLds<M, NW> L;
+  constexpr bool ONE = step_one_barrier<M>();
+  __shared__ StepLds<M, NW, ONE ? 2 : 1> L;
   const StepTileShape T = step_tile_shape<NW>(xcd_remap(blockIdx.x, gridDim.x), P.n1, tile_h,
                                               nfull);
   TileOut res{0, 0.0f, 0.0f};
-  step_tile<M, NW, STEP_SLAB>(uin, uout, s, P, g0, g1, g2, b1, b2, T.j0, T.i0, T.i1, T.lane0,
-                              T.wl, T.nit, cfl, L, res, H);
+  step_tile<M, NW, STEP_SLAB, ONE>(uin, uout, s, P, g0, g1, g2, b1, b2, T.j0, T.i0, T.i1,
+                                   T.lane0, T.wl, T.nit, cfl, L, res, H);
   commit_tile(res, cfl, diag);
 }
 
@@ -742,12 +824,13 @@ void bounded_step_kernel(const float *__restrict__ uin, float *__restrict__ uout
                          const float b1, const float b2, const int tile_h, const int nfull,
                          const int cfl, DeviceDiag *diag)
 {
-  __shared__ StepLds<M, NW> L;
+  constexpr bool ONE = step_one_barrier<M>();
+  __shared__ StepLds<M, NW, ONE ? 2 : 1> L;
   const StepTileShape T = step_tile_shape<NW>(xcd_remap(blockIdx.x, gridDim.x), P.n1, tile_h,
                                               nfull);
   TileOut res{0, 0.0f, 0.0f};
-  step_tile<M, NW, STEP_BOUNDED>(uin, uout, s, P, g0, g1, g2, b1, b2, T.j0, T.i0, T.i1, T.lane0,
-                                 T.wl, T.nit, cfl, L, res);
+  step_tile<M, NW, STEP_BOUNDED, ONE>(uin, uout, s, P, g0, g1, g2, b1, b2, T.j0, T.i0, T.i1,
+                                      T.lane0, T.wl, T.nit, cfl, L, res);
   commit_tile(res, cfl, diag);
 }
 

@@ -79,6 +79,7 @@ struct Disk {
   // and D2 of every variable, D11 and D22 of all but Omg and lne, whose
   // viscous terms need only their Laplacian, and D12 of the D12V variables.
   static constexpr bool DERIV_FIRST = true;
+  static constexpr bool STEP_ONE_BARRIER = false;   // no one-pass step (ONE_PASS)
   enum { LND, UR, UZ, OMG, LNE, LNER };
   struct Deriv { float a[NV], b[NV], d11[NV], d22[NV], lap[NV], d12[ND]; };
   static __device__ __forceinline__ void deriv_var(const Params &P, const P_t &x, int q, Deriv &D)

@@ -283,16 +283,24 @@ FusedShape shape_of(const fg2_run *r)
 
 FusedShape step_shape_of(const fg2_run *r);
 
-// A grid that fills under half of one round of the one-pass kernel's
-// workgroups is latency bound: a one-pass tile takes tile_h + 12 dependent
-// iterations of three stages, while the substage path's one- or two-row
-// tiles take three short launches.  Hydro 128^2, 256^2, 512^2 step in 10.2,
-// 12.0 and 18.1 us on the substage path against 12.9, 15.9 and 21.1 us on
-// the one-pass kernel's small-grid shape.
+// A grid for which the one-pass kernel takes its small-grid shape, or its
+// shortest tiles (12 rows) filling at most three quarters of one round of
+// resident workgroups, is latency bound: a one-pass tile takes tile_h + 12
+// or more dependent iterations of three stages, while the substage path's
+// short tiles take three short launches.  Per step, substage against
+// one-pass: hydro 128^2 10.2 / 12.9 us, 512^2 18.1 / 22.0 us, 896^2 (12-row
+// tiles, 0.59 of a round) 32.5 / 38.6 us; hydro-isot 1024^2 (0.56) 31.0 /
+// 33.9 us.  Fuller rounds go the other way: hydro 1024^2 (0.84) 38.8 / 39.2
+// us, hydro-isot 1280^2 (0.84) 44.5 / 40.0 us.
 bool latency_bound(const fg2_run *r)
 {
   if(!r->tile_auto || r->slab_n1g) return false;
-  return step_shape_of(r).waves == 1;
+  const FusedShape sh = step_shape_of(r);
+  if(sh.waves == 1) return true;
+  if(sh.tile_h > 12) return false;
+  const int kind = (r->p1 && r->p2) ? 1 : 4;
+  const long slots = (long)blocks_per_cu(r->module, kind, sh.waves) * r->cus;
+  return 4 * step_launch_tiles(r->n1, r->n2, sh.tile_h, sh.waves, nullptr) <= 3 * slots;
 }
 
 // The one-pass step kernel serves the Cartesian modules, periodic or not,

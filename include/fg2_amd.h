@@ -49,8 +49,10 @@ enum {
 /* Kernel of the fused path for hydro and hydro-isot (fg2_set_kernel). */
 enum {
   FG2_KERNEL_AUTO     = 0, /* default: the one-pass step kernel, except that a grid
-                              too small to fill half of the GPU with one-pass tiles
-                              and an automatic tile takes the substage kernels     */
+                              too small to fill most of the GPU with one-pass
+                              tiles (hydro up to about 900^2, hydro-isot up to
+                              about 1100^2) with an automatic tile takes the
+                              substage kernels                                     */
   FG2_KERNEL_ONE_PASS = 1, /* the one-pass step kernel, whatever the grid size     */
   FG2_KERNEL_SUBSTAGE = 2  /* one fused kick+drift launch per substage             */
 };
@@ -194,8 +196,8 @@ int fg2_set_dt_mode(fg2_run *run, int mode);
  * chunk; the substage kernel gives each wavefront 58).  With the automatic
  * kernel choice, a tile set here steps a module that has the one-pass
  * kernel with that kernel; tile_h = 0 restores the automatic shape, which
- * steps a grid too small to fill half of the GPU with one-pass tiles (hydro
- * up to about 512^2) with the substage kernels and one- or two-row tiles. */
+ * steps a grid too small to fill most of the GPU with one-pass tiles
+ * (hydro up to about 900^2) with the substage kernels and short tiles. */
 int fg2_set_tile(fg2_run *run, int tile_h, int waves);
 int fg2_get_tile(const fg2_run *run, int *tile_h, int *waves);
 /* Fix the fused path's kernel of a hydro or hydro-isot run (FG2_KERNEL_*;

@@ -25,9 +25,10 @@ host cores, on a bounded sample of the same workload).
 The other GPU legs run before the headline, and the headline's own
 substage-path leg (100 steps of the same workload) is queued back to back
 with its warm-up and timed steps: after 5 ms or more without work the step
-takes 250-290 us per launch, and it returns to its steady 235 us only after
-some 100 launches (25 ms) of sustained load (the GPU's clocks ramp up), so a
-headline of 20 steps after an idle gap would time the ramp.
+takes 250-290 us per launch (measured with one barrier per stage), and it
+returns to its steady launch time only after some 100 launches (25 ms) of
+sustained load (the GPU's clocks ramp up), so a headline of 20 steps after
+an idle gap would time the ramp.
 """
 import argparse
 import ctypes
@@ -527,12 +528,13 @@ def bench_config1(ev, steps=100, reps=20):
         best = ms if best is None else min(best, ms)
     status = run.check()
     tile_h, waves = run.tile()
+    kernel = "one_pass" if run.kernel() == fg2_amd.KERNEL_ONE_PASS else "substage"
     run.close()
     return {"workload": f"hydro 256x256 random velocity, {steps} RK3 steps on the GPU "
                         f"(least of {reps} repetitions from the same state)",
             "ms_total": best, "us_per_step": 1e3 * best / steps,
             "cell_updates_per_s": 256 * 256 * steps / (best * 1e-3), "status": status,
-            "tile": {"tile_h": tile_h, "waves": waves}}
+            "tile": {"tile_h": tile_h, "waves": waves}, "kernel": kernel}
 
 
 def cpu_config1(steps=100):
@@ -569,7 +571,9 @@ def bench_sweep(ev, sides=(128, 256, 512, 1024, 2048, 3584, 4096, 8192, 14336), 
     """hydro-isot on n x n periodic grids from the device initial condition
     Kf (the module's own IC, SplitMix64 noise), the automatic launch shape,
     closed-form dt: ms per RK3 step at each side, the reference's M2090
-    figure where it publishes one, and the one-pass kernel's tile."""
+    figure where it publishes one, and the kernel the automatic choice takes
+    (the substage kernels on grids too small for the one-pass kernel) with
+    its tile."""
     import fg2_amd
     out = []
     for n in sides:
@@ -588,7 +592,8 @@ def bench_sweep(ev, sides=(128, 256, 512, 1024, 2048, 3584, 4096, 8192, 14336), 
         ms = ev.elapsed_ms(e0, e1) / steps
         tile_h, waves = run.tile()
         row = {"n": n, "ms_per_step": ms, "cell_updates_per_s": n * n / (ms * 1e-3),
-               "status": run.check(), "tile": {"tile_h": tile_h, "waves": waves}}
+               "status": run.check(), "tile": {"tile_h": tile_h, "waves": waves},
+               "kernel": "one_pass" if run.kernel() == fg2_amd.KERNEL_ONE_PASS else "substage"}
         if n in REF_ISOT_M2090_MS:
             row["reference_m2090_ms_per_step"] = REF_ISOT_M2090_MS[n]
             row["vs_reference_m2090"] = REF_ISOT_M2090_MS[n] / ms

@@ -133,6 +133,7 @@ def _load():
         "fg2_set_tile": (I, [P, I, I]),
         "fg2_get_tile": (I, [P, IP, IP]),
         "fg2_set_kernel": (I, [P, I]),
+        "fg2_get_kernel": (I, [P, IP]),
         "fg2_autotune": (I, [P, IP, IP]),
         "fg2_step": (I, [P, D, D]),
         "fg2_step_async": (I, [P, D, D]),
@@ -296,6 +297,13 @@ class Run:
         """Fix the fused path's kernel (KERNEL_AUTO, KERNEL_ONE_PASS,
         KERNEL_SUBSTAGE) of a hydro or hydro-isot run."""
         _check(lib.fg2_set_kernel(self.h, int(kernel)))
+
+    def kernel(self):
+        """The kernel a step takes now: KERNEL_ONE_PASS, KERNEL_SUBSTAGE, or
+        0 on the plain path."""
+        k = ctypes.c_int()
+        _check(lib.fg2_get_kernel(self.h, ctypes.byref(k)))
+        return k.value
 
     def autotune(self):
         a, b = ctypes.c_int(), ctypes.c_int()

@@ -788,6 +788,14 @@ int fg2_set_kernel(fg2_run *r, int kernel)
   return FG2_OK;
 }
 
+int fg2_get_kernel(const fg2_run *r, int *kernel)
+{
+  if(!r || !kernel) return fail(FG2_ERR_ARG, "NULL argument");
+  *kernel = uses_step_kernel(r) ? FG2_KERNEL_ONE_PASS
+                                : r->path == FG2_PATH_PLAIN ? FG2_KERNEL_AUTO : FG2_KERNEL_SUBSTAGE;
+  return FG2_OK;
+}
+
 int fg2_get_tile(const fg2_run *r, int *tile_h, int *waves)
 {
   if(!r) return fail(FG2_ERR_ARG, "NULL run");

@@ -206,6 +206,10 @@ int fg2_get_tile(const fg2_run *run, int *tile_h, int *waves);
  * the substage kernels; many small runs launched side by side on their own
  * streams go faster on the one-pass kernel, one launch per step. */
 int fg2_set_kernel(fg2_run *run, int kernel);
+/* The kernel a step of the run takes now: FG2_KERNEL_ONE_PASS or
+ * FG2_KERNEL_SUBSTAGE (disk, FG2_PATH_SUBSTAGE, or the automatic choice on
+ * a small grid); 0 on the plain path.  New, as fg2_set_kernel. */
+int fg2_get_kernel(const fg2_run *run, int *kernel);
 /* Time the run's fused kernel on its own state for candidate tile shapes
  * and keep the fastest (cached per module and grid size for the process);
  * replaces the block-size heuristic of setup() (src/setup.cpp:39-58).  With

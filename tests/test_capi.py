@@ -47,6 +47,7 @@ def test_error_paths_without_device():
     assert L.fg2_config(None) == fg2_amd.ERR_ARG
     assert L.fg2_step(None, 0.0, 1.0) == fg2_amd.ERR_ARG
     assert L.fg2_set_kernel(None, fg2_amd.KERNEL_AUTO) == fg2_amd.ERR_ARG
+    assert L.fg2_get_kernel(None, None) == fg2_amd.ERR_ARG
 
 
 def test_product_does_not_import_the_oracle():

@@ -321,6 +321,7 @@ def test_small_grid_takes_short_substage_tiles(fg2, module):
     a = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED)
     th, w = a.tile()
     assert th <= 2 and w == 1, (th, w)
+    assert a.kernel() == fg2.KERNEL_SUBSTAGE
     assert a.run(0.0, dt, 3) == 0
     np.testing.assert_array_equal(a.download(), ref.download())
     b = _run(fg2, module, n1, n2, x, fg2.PATH_FUSED)
@@ -333,9 +334,9 @@ def test_small_grid_takes_short_substage_tiles(fg2, module):
     assert c.run(0.0, dt, 3) == 0
     np.testing.assert_array_equal(c.download(), ref.download())
     big = fg2.Run(module, 2048, 2048)
-    assert big.tile()[1] == 4
+    assert big.tile()[1] == 4 and big.kernel() == fg2.KERNEL_ONE_PASS
     big.set_kernel(fg2.KERNEL_SUBSTAGE)
-    assert big.tile()[1] == 1
+    assert big.tile()[1] == 1 and big.kernel() == fg2.KERNEL_SUBSTAGE
     d = fg2.Run("disk", 64, 64, params=params_for("disk"))
     with pytest.raises(fg2.Fg2Error):
         d.set_kernel(fg2.KERNEL_ONE_PASS)

@@ -338,8 +338,11 @@ def test_small_grid_takes_short_substage_tiles(fg2, module):
     big.set_kernel(fg2.KERNEL_SUBSTAGE)
     assert big.tile()[1] == 1 and big.kernel() == fg2.KERNEL_SUBSTAGE
     d = fg2.Run("disk", 64, 64, params=params_for("disk"))
+    assert d.kernel() == fg2.KERNEL_SUBSTAGE
     with pytest.raises(fg2.Fg2Error):
         d.set_kernel(fg2.KERNEL_ONE_PASS)
+    plain = fg2.Run(module, 64, 64, path=fg2.PATH_PLAIN)
+    assert plain.kernel() == 0
 
 
 @pytest.mark.parametrize("waves,tile_h", [(4, 48), (0, 0)])

@@ -390,8 +390,8 @@ void substage_kernel(const float *__restrict__ uin, float *__restrict__ uout, fl
 // each stage exactly the per-cell arithmetic of the substage kernel.  The
 // stage v's travel in small register rings (V0, V1); only u is read and the
 // new u written, so HBM traffic per step is 2 NVAR floats per cell.  (The
-// one-barrier schedule, below, lags stages 1 and 2 by four rows each.)  Rows and
-// columns outside the grid wrap (periodic), and the right-hand side sees the
+// one-barrier schedule, below, lags stages 1 and 2 by four rows each.)  Rows
+// and columns outside the grid wrap (periodic), and the right-hand side sees the
 // wrapped indices, so every halo value equals the value the substage path
 // computes at the wrapped cell, bit for bit.  Horizontal neighbours of a
 // stage's centre row come from LDS: one s_barrier per stage, or one per
@@ -535,8 +535,9 @@ __device__ __forceinline__ void step_tile(const float *__restrict__ uin, float *
   // lanes tl <= n2 - j0 + 17 (its last output column plus the 9-column halo).
   // Wavefronts wholly past that, and wavefronts of a packed narrow strip
   // without rows, skip the work and keep only the workgroup's barriers,
-  // three (or one) per iteration as below.  The lanes next to them read stale LDS,
-  // which only reaches lanes more than 9 columns past the last output column.
+  // three (or one) per iteration as below.  The lanes next to them read
+  // stale LDS, which only reaches lanes more than 9 columns past the last
+  // output column.
   if(64 * __builtin_amdgcn_readfirstlane(t >> 6) - lane0 > n2 - j0 + 2 * H3 - 1 || i1 <= i0) {
     for(int it = 0; it < nit; ++it) {
       __syncthreads();

@@ -334,6 +334,18 @@ __device__ __forceinline__ float wave_max(float x)
 // Fold one tile's NaN flag and CFL rates into the run's diagnostics.
 // (device-scope relaxed atomics, as atomicOr / atomicMax, through global
 // pointers: the diagnostics live in device memory)
+// Fold a non-negative double's bits into a running maximum in device
+// memory (non-negative doubles order like their bit patterns): a relaxed
+// load, and the atomic only when the value is larger, so that the blocks
+// whose maximum is not (most of them) do not queue atomics on one address.
+__device__ __forceinline__ void max_bits(unsigned long long *p, const double x)
+{
+  FG2_GLOBAL unsigned long long *g = (FG2_GLOBAL unsigned long long *)p;
+  const unsigned long long v = (unsigned long long)__double_as_longlong(x);
+  if(v > __hip_atomic_load(g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    __hip_atomic_fetch_max(g, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void commit_tile(const TileOut &r, const int cfl, DeviceDiag *diag)
 {
   const unsigned long long badmask = __ballot(r.bad);
@@ -344,13 +356,8 @@ __device__ __forceinline__ void commit_tile(const TileOut &r, const int cfl, Dev
   if(cfl) {
     const float la = wave_max(r.lam_adv), ld = wave_max(r.lam_diff);
     if(lane == 0) {
-      // non-negative doubles order like their bit patterns
-      __hip_atomic_fetch_max((FG2_GLOBAL unsigned long long *)&diag->lam_adv_bits,
-                             (unsigned long long)__double_as_longlong((double)la),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_max((FG2_GLOBAL unsigned long long *)&diag->lam_diff_bits,
-                             (unsigned long long)__double_as_longlong((double)ld),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      max_bits(&diag->lam_adv_bits, (double)la);
+      max_bits(&diag->lam_diff_bits, (double)ld);
     }
   }
 }
@@ -1019,21 +1026,37 @@ void euler_kernel(float *u, const long s, const Params P, const float dt)
 
 // CFL rates of u for a state no fused stage-2 epilogue has reduced (the
 // first step, or after an upload): columns across threads, rows strided
-// over gridDim.y, one block reduction and one atomic per block.
+// over gridDim.y, four rows' loads in flight per thread, one block
+// reduction, and an atomic only from a block that raises the maximum.  (max is exact, so the order of the
+// rows does not change the result.)
 template<class M>
 __global__ __launch_bounds__(256)
 void cfl_kernel(const float *u, const long s, const Params P, DeviceDiag *diag)
 {
+  constexpr int NV = M::NV;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const float k1a = 1.5859783962656393f / P.Delta1, k1b = 1.5859783962656393f / P.Delta2;
   float la = 0.0f, ld = 0.0f;
-  if(j < P.n2)
-    for(int i = blockIdx.y; i < P.n1; i += gridDim.y) {
-      float c[M::NV];
-      load_cell<M::NV>(u + (long)i * s + (long)j * M::NV, c);
-      la = fmaxf(la, M::lam_adv(P, c, i, j, k1a, k1b));
-      if constexpr(M::CELL_DIFF) ld = fmaxf(ld, M::lam_diff(P, c, i, j));
+  auto fold = [&](const float (&c)[NV], int i) {
+    la = fmaxf(la, M::lam_adv(P, c, i, j, k1a, k1b));
+    if constexpr(M::CELL_DIFF) ld = fmaxf(ld, M::lam_diff(P, c, i, j));
+  };
+  if(j < P.n2) {
+    const int gy = gridDim.y;
+    int i = blockIdx.y;
+    for(; i + 3 * gy < P.n1; i += 4 * gy) {
+      float c[4][NV];
+#pragma unroll
+      for(int k = 0; k < 4; ++k) gload_cell<NV>(gptr(u + (long)(i + k * gy) * s + (long)j * NV), c[k]);
+#pragma unroll
+      for(int k = 0; k < 4; ++k) fold(c[k], i + k * gy);
     }
+    for(; i < P.n1; i += gy) {
+      float c[NV];
+      gload_cell<NV>(gptr(u + (long)i * s + (long)j * NV), c);
+      fold(c, i);
+    }
+  }
   la = wave_max(la);
   ld = wave_max(ld);
   __shared__ float red[2][4];
@@ -1042,8 +1065,8 @@ void cfl_kernel(const float *u, const long s, const Params P, DeviceDiag *diag)
   __syncthreads();
   if(threadIdx.x == 0) {
     for(int k = 1; k < 4; ++k) { la = fmaxf(la, red[0][k]); ld = fmaxf(ld, red[1][k]); }
-    atomicMax(&diag->lam_adv_bits, (unsigned long long)__double_as_longlong((double)la));
-    atomicMax(&diag->lam_diff_bits, (unsigned long long)__double_as_longlong((double)ld));
+    max_bits(&diag->lam_adv_bits, (double)la);
+    max_bits(&diag->lam_diff_bits, (double)ld);
   }
 }
 
@@ -1208,8 +1231,9 @@ hipError_t launch_cfl(int module, const float *u, long s, const Params &P, Devic
 {
   return by_module(module, [&](auto m) {
     using M = decltype(m);
+    // about 4096 blocks: 12-13 rows per thread at 3584^2
     const int gx = (P.n2 + 255) / 256;
-    const int gy = std::max(1, std::min(P.n1, 2048 / gx));
+    const int gy = std::max(1, std::min(P.n1, 4096 / gx));
     cfl_kernel<M><<<dim3(gx, gy), 256, 0, st>>>(u, s, P, diag);
     return hipGetLastError();
   });

@@ -16,8 +16,8 @@ struct HydroIsot {
                         RANDOM_PHASES = true, CELL_DIFF = false;
   // the right-hand side takes the stencil primitives whole (rhs)
   static constexpr bool DERIV_FIRST = false;
-  // the one-pass step syncs its workgroup once per stage: once per row
-  // would cost it its third wavefront per SIMD and 17%
+  // the periodic one-pass step syncs its workgroup once per stage: once per
+  // row would cost it its third wavefront per SIMD and 17%
   static constexpr bool STEP_ONE_BARRIER = false;
   using P_t = Prim<NV, ND>;
 

@@ -21,9 +21,10 @@
 //                 mixed ones) and rhs_d() (the rest, from those), so that
 //                 the substage kernel keeps derivatives, not stencil
 //                 primitives, live through the right-hand side
-//   STEP_ONE_BARRIER  the one-pass step's schedule: one workgroup barrier
-//                 per row (stages lagged by four rows, double-buffered LDS
-//                 rows) or one per stage (fewer registers)
+//   STEP_ONE_BARRIER  the one-pass step's schedule on periodic grids and
+//                 slabs: one workgroup barrier per row (stages lagged by four
+//                 rows, double-buffered LDS rows) or one per stage (fewer
+//                 registers); the bounded kernel always takes one per row
 //   first_order() and first_order_on(), HAS_FIRST_ORDER, RANDOM_PHASES
 //                 the point-wise pre-step, when it runs, and whether it
 //                 draws two random phases per step                  (Euler.cuh)
